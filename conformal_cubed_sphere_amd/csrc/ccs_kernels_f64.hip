@@ -1,0 +1,420 @@
+// ccs_kernels_f64.hip — FP64 parity kernels for gfx950.
+//
+// One launch performs one Runge-Kutta stage of the reference time step:
+//   spatial_operator (spatial_operators_mod.F90:13-181)
+//   + the stage update (time_scheme_mod.F90:69-115)
+//   + correct_bdy_ghost (time_scheme_mod.F90:117-125): the interior
+//     covariant -> sphere -> covariant round trip and the edge-halo exchange,
+//   so each stage reads the input state once and writes the output state once.
+//
+// Arithmetic follows the reference expression by expression (this file is
+// compiled with -ffp-contract=off; divisions and square roots are the IEEE
+// correctly rounded ones), which keeps the branchy WENO-Z weight selection
+// (spatial_operators_mod.F90:348-355) on the same side as the CPU reference.
+//
+// Tiling: a 256-thread workgroup owns a TX x TY tile of one panel.  Phase 1
+// evaluates the pointwise terms (phi, contravariant winds, Bernoulli function,
+// mass fluxes) on the tile plus a 3-cell cross-shaped border into LDS.
+// Phase 2 evaluates every x- and y-face of the tile once (six WENO-Z
+// reconstructions and the Rusanov flux per face).  Phase 3 forms the
+// divergence, the 6th-order vorticity and the stage update per cell, writes
+// the cell and scatters edge-strip cells into the halos of the neighbouring
+// panels (the inverse of ghost_mod.f90:34-76), rotating the wind through the
+// destination panel's basis as convert_wind_SP2P does.
+#include "ccs_kernels.h"
+
+namespace ccs {
+namespace {
+
+constexpr int TX = 32;
+constexpr int TY = 8;
+constexpr int NT = TX * TY;
+constexpr int EX = TX + 6;
+constexpr int EY = TY + 6;
+constexpr int NE = EX * EY;
+
+__device__ __forceinline__ long gidx(int i, int j, int p, int h, int P) {
+  return static_cast<long>(i + h) + static_cast<long>(P) * (static_cast<long>(j + h) + static_cast<long>(P) * p);
+}
+
+// WENO_limiter with dir > 0 on the window (q1..q5) centred on q3
+// (spatial_operators_mod.F90:282-361).  dir < 0 is the same expression on the
+// reversed window, weno5(q5,q4,q3,q2,q1).
+__device__ __forceinline__ double weno5(double q1, double q2, double q3, double q4, double q5) {
+  const double s1 = q1 / 3. - 7. / 6. * q2 + 11. / 6. * q3;
+  const double s2 = -(q2 / 6.) + 5. / 6. * q3 + 1. / 3. * q4;
+  const double s3 = q3 / 3. + 5. / 6. * q4 - 1. / 6. * q5;
+  const double a1 = q1 - 2. * q2 + q3;
+  const double a2 = q2 - 2. * q3 + q4;
+  const double a3 = q3 - 2. * q4 + q5;
+  const double b1 = q1 - 4. * q2 + 3. * q3;
+  const double b2 = q2 - q4;
+  const double b3 = 3. * q3 - 4. * q4 + q5;
+  const double be1 = a1 * a1 * 13. / 12. + b1 * b1 * 0.25;
+  const double be2 = a2 * a2 * 13. / 12. + b2 * b2 * 0.25;
+  const double be3 = a3 * a3 * 13. / 12. + b3 * b3 * 0.25;
+  const double t40 = fabs(be1 - be2);
+  const double t41 = fabs(be2 - be3);
+  const double t5 = fabs(be3 - be1);
+  double mb = be1;
+  if (be2 < mb) mb = be2;
+  if (be3 < mb) mb = be3;
+  double w1, w2, w3;
+  if (t40 <= mb && t41 > mb) {
+    w1 = 1. / 3.; w2 = 2. / 3.; w3 = 0.;
+  } else if (t40 > mb && t41 < mb) {
+    w1 = 0.; w2 = 2. / 3.; w3 = 1. / 3.;
+  } else {
+    const double al1 = 0.1 * (1. + t5 / (1.E-2 + be1));
+    const double al2 = 0.6 * (1. + t5 / (1.E-2 + be2));
+    const double al3 = 0.3 * (1. + t5 / (1.E-2 + be3));
+    const double sa = al1 + al2 + al3;
+    w1 = al1 / sa; w2 = al2 / sa; w3 = al3 / sa;
+  }
+  return s1 * w1 + s2 * w2 + s3 * w3;
+}
+
+__device__ __forceinline__ double d6(double m3, double m2, double m1, double p1, double p2, double p3,
+                                     double dh) {
+  return (-m3 + 9. * m2 - 45. * m1 + 45. * p1 - 9. * p2 + p3) / (60. * dh);
+}
+
+template <int KIND, bool EXPL>
+__global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, HaloScatter hs) {
+  // LDS fields on the extended tile (row-major [lj][li])
+  __shared__ double sFx[NE], sFy[NE], sE[NE], sPtG[NE], sUC[NE], sVC[NE], sPt[NE], sG11[NE],
+      sG22[NE], sU[NE], sV[NE];
+  __shared__ double sFxF[TY * (TX + 1)], sExF[TY * (TX + 1)];
+  __shared__ double sFyF[(TY + 1) * TX], sEyF[(TY + 1) * TX];
+
+  const int n = a.n, h = a.h, P = n + 2 * h;
+  const int p = blockIdx.z;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int tid = threadIdx.x;
+
+  // ---- phase 1: pointwise terms on the cross-shaped extended tile
+  constexpr int NX = EX * TY;       // rows of the tile, all extended columns
+  constexpr int NCROSS = NX + 6 * TX;
+  for (int e = tid; e < NCROSS; e += NT) {
+    int li, lj;
+    if (e < NX) {
+      li = e % EX; lj = 3 + e / EX;
+    } else {
+      const int e2 = e - NX;
+      li = 3 + e2 % TX;
+      const int r = e2 / TX;
+      lj = (r < 3) ? r : TY + r;
+    }
+    const int i = i0 + li - 3, j = j0 + lj - 3;
+    if (i < -h || i >= n + h || j < -h || j >= n + h) continue;
+    if ((i < 0 || i >= n) && (j < 0 || j >= n)) continue;  // panel corner: never read
+    const long g = gidx(i, j, p, h, P);
+    const double phiG = a.in.phiG[g], u = a.in.u[g], v = a.in.v[g];
+    const double sg = st.sqrtG[g];
+    const double g11 = st.IG11[g], g22 = st.IG22[g];
+    double phi, uc, vc;
+    if (EXPL) {
+      phi = a.phi0[g]; uc = a.uc0[g]; vc = a.vc0[g];
+    } else {
+      phi = phiG / sg;                           // time_scheme_mod.F90:81
+      uc = g11 * u + st.IG12[g] * v;             // projection_mod.F90:145-146
+      vc = st.IG21[g] * u + g22 * v;
+    }
+    const double phit = phi + st.phis[g];        // spatial_operators_mod.F90:71-78
+    const double phitG = phit * sg;
+    const double K = 0.5 * (u * uc + v * vc);
+    const int l = lj * EX + li;
+    sE[l] = phit + K;
+    sFx[l] = phiG * uc;
+    sFy[l] = phiG * vc;
+    sPtG[l] = phitG;
+    sPt[l] = phit;
+    sUC[l] = uc;
+    sVC[l] = vc;
+    sG11[l] = g11;
+    sG22[l] = g22;
+    sU[l] = u;
+    sV[l] = v;
+  }
+  __syncthreads();
+
+  // ---- phase 2: faces (spatial_operators_mod.F90:80-149)
+  constexpr int NFX = (TX + 1) * TY;
+  constexpr int NFY = TX * (TY + 1);
+  for (int e = tid; e < NFX + NFY; e += NT) {
+    if (e < NFX) {
+      const int fi = e % (TX + 1), ty = e / (TX + 1);
+      const int ig = i0 + fi - 1, j = j0 + ty;
+      if (j >= n || ig < -1 || ig > n - 1) continue;
+      const int r = (ty + 3) * EX;
+      const int c = fi + 2;  // LDS column of cell ig
+      const double* F = sFx + r;
+      const double* E = sE + r;
+      const double* Q = sPtG + r;
+      const double Fp = weno5(F[c - 2], F[c - 1], F[c], F[c + 1], F[c + 2]);
+      const double Fn = weno5(F[c + 3], F[c + 2], F[c + 1], F[c], F[c - 1]);
+      const double Ep = weno5(E[c - 2], E[c - 1], E[c], E[c + 1], E[c + 2]);
+      const double En = weno5(E[c + 3], E[c + 2], E[c + 1], E[c], E[c - 1]);
+      const double Pp = weno5(Q[c - 2], Q[c - 1], Q[c], Q[c + 1], Q[c + 2]);
+      const double Pn = weno5(Q[c + 3], Q[c + 2], Q[c + 1], Q[c], Q[c - 1]);
+      const double uB = 0.5 * (sUC[r + c + 1] + sUC[r + c]);
+      const double pB = 0.5 * (sPt[r + c + 1] + sPt[r + c]);
+      const double gq = 0.5 * (sG11[r + c + 1] + sG11[r + c]) * pB;
+      const double e1 = uB - sqrt(gq), e2 = uB + sqrt(gq);
+      const double lam = fabs(e1) > fabs(e2) ? fabs(e1) : fabs(e2);
+      sFxF[ty * (TX + 1) + fi] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
+      sExF[ty * (TX + 1) + fi] = 0.5 * (Ep + En - lam * (0. - 0.));
+    } else {
+      const int e2 = e - NFX;
+      const int tx = e2 % TX, fj = e2 / TX;
+      const int i = i0 + tx, jg = j0 + fj - 1;
+      if (i >= n || jg < -1 || jg > n - 1) continue;
+      const int c = tx + 3;
+      const int r = fj + 2;  // LDS row of cell jg
+      auto at = [c](const double* s, int row) { return s[row * EX + c]; };
+      const double Fp = weno5(at(sFy, r - 2), at(sFy, r - 1), at(sFy, r), at(sFy, r + 1), at(sFy, r + 2));
+      const double Fn = weno5(at(sFy, r + 3), at(sFy, r + 2), at(sFy, r + 1), at(sFy, r), at(sFy, r - 1));
+      const double Ep = weno5(at(sE, r - 2), at(sE, r - 1), at(sE, r), at(sE, r + 1), at(sE, r + 2));
+      const double En = weno5(at(sE, r + 3), at(sE, r + 2), at(sE, r + 1), at(sE, r), at(sE, r - 1));
+      const double Pp = weno5(at(sPtG, r - 2), at(sPtG, r - 1), at(sPtG, r), at(sPtG, r + 1), at(sPtG, r + 2));
+      const double Pn = weno5(at(sPtG, r + 3), at(sPtG, r + 2), at(sPtG, r + 1), at(sPtG, r), at(sPtG, r - 1));
+      const double vB = 0.5 * (at(sVC, r + 1) + at(sVC, r));
+      const double pB = 0.5 * (at(sPt, r + 1) + at(sPt, r));
+      const double gq = 0.5 * (at(sG22, r + 1) + at(sG22, r)) * pB;
+      const double e1 = vB - sqrt(gq), e2v = vB + sqrt(gq);
+      const double lam = fabs(e1) > fabs(e2v) ? fabs(e1) : fabs(e2v);
+      sFyF[fj * TX + tx] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
+      sEyF[fj * TX + tx] = 0.5 * (Ep + En - lam * (0. - 0.));
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 3: cells
+  const int tx = tid % TX, ty = tid / TX;
+  const int i = i0 + tx, j = j0 + ty;
+  if (i >= n || j >= n) return;
+  const double dx = a.dx;
+  const int lc = (ty + 3) * EX + (tx + 3);
+  const double dfluxdx = (sFxF[ty * (TX + 1) + tx + 1] - sFxF[ty * (TX + 1) + tx]) / dx;
+  const double dEdx = (sExF[ty * (TX + 1) + tx + 1] - sExF[ty * (TX + 1) + tx]) / dx;
+  const double dvdx = d6(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], dx);
+  const double dfluxdy = (sFyF[(ty + 1) * TX + tx] - sFyF[ty * TX + tx]) / dx;
+  const double dEdy = (sEyF[(ty + 1) * TX + tx] - sEyF[ty * TX + tx]) / dx;
+  const double dudy = d6(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX],
+                         sU[lc + 3 * EX], dx);
+  const long g = gidx(i, j, p, h, P);
+  const double vort = dvdx - dudy + st.sqrtG[g] * st.f[g];
+  const double tp = -(dfluxdx + dfluxdy);
+  const double tu = -dEdx + vort * sVC[lc];
+  const double tv = -dEdy - vort * sUC[lc];
+
+  const long ii = static_cast<long>(i) + static_cast<long>(n) * (static_cast<long>(j) + static_cast<long>(n) * p);
+  if (KIND == kTendOnly) {
+    a.tend[0][ii] = tp;
+    a.tend[1][ii] = tu;
+    a.tend[2][ii] = tv;
+    return;
+  }
+  double pn, un, vn;
+  if (KIND == kRK3s1) {
+    pn = a.base.phiG[g] + a.dt * tp;
+    un = a.base.u[g] + a.dt * tu;
+    vn = a.base.v[g] + a.dt * tv;
+  } else if (KIND == kRK3s2) {
+    pn = 0.75 * a.base.phiG[g] + 0.25 * a.in.phiG[g] + 0.25 * a.dt * tp;
+    un = 0.75 * a.base.u[g] + 0.25 * a.in.u[g] + 0.25 * a.dt * tu;
+    vn = 0.75 * a.base.v[g] + 0.25 * a.in.v[g] + 0.25 * a.dt * tv;
+  } else if (KIND == kRK3s3) {
+    pn = a.base.phiG[g] / 3. + 2. / 3. * a.in.phiG[g] + 2. / 3. * a.dt * tp;
+    un = a.base.u[g] / 3. + 2. / 3. * a.in.u[g] + 2. / 3. * a.dt * tu;
+    vn = a.base.v[g] / 3. + 2. / 3. * a.in.v[g] + 2. / 3. * a.dt * tv;
+  } else if (KIND == kRK4a) {
+    a.tend[0][ii] = tp;
+    a.tend[1][ii] = tu;
+    a.tend[2][ii] = tv;
+    pn = a.base.phiG[g] + a.inc * tp;
+    un = a.base.u[g] + a.inc * tu;
+    vn = a.base.v[g] + a.inc * tv;
+  } else {  // kRK4f
+    const double T0 = (a.tacc[0][ii] + 2. * a.tacc[3][ii] + 2. * a.tacc[6][ii] + tp) / 6.;
+    const double T1 = (a.tacc[1][ii] + 2. * a.tacc[4][ii] + 2. * a.tacc[7][ii] + tu) / 6.;
+    const double T2 = (a.tacc[2][ii] + 2. * a.tacc[5][ii] + 2. * a.tacc[8][ii] + tv) / 6.;
+    pn = a.base.phiG[g] + a.dt * T0;
+    un = a.base.u[g] + a.dt * T1;
+    vn = a.base.v[g] + a.dt * T2;
+  }
+  // correct_bdy_ghost: convert_wind_P2SP on the interior cell ...
+  const double zw = st.M11[g] * un + st.M12[g] * vn;
+  const double mw = st.M21[g] * un + st.M22[g] * vn;
+  // ... convert_wind_SP2P back on the same cell
+  a.out.phiG[g] = pn;
+  a.out.u[g] = st.N11[g] * zw + st.N12[g] * mw;
+  a.out.v[g] = st.N21[g] * zw + st.N22[g] * mw;
+  // ... and the halo copies of (phiG, zonal, meridional) rotated into the
+  // destination panel basis.
+  const int i1 = i + 1, j1 = j + 1;
+  const bool strip[4] = {i1 <= h, i1 >= n - h + 1, j1 <= h, j1 >= n - h + 1};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (!strip[s]) continue;
+    const int* r = hs.r[p][s];
+    const int pd = r[0] - 1;
+    const int id = r[1] + r[2] * i1 + r[3] * j1;
+    const int jd = r[4] + r[5] * i1 + r[6] * j1;
+    const long gd = gidx(id - 1, jd - 1, pd, h, P);
+    a.out.phiG[gd] = pn;
+    a.out.u[gd] = st.N11[gd] * zw + st.N12[gd] * mw;
+    a.out.v[gd] = st.N21[gd] * zw + st.N22[gd] * mw;
+  }
+}
+
+__global__ void derived_f64(Prog<const double> q, StaticsF64 st, int n, int h, double* phi, double* uc,
+                            double* vc, double* zonal, double* merid) {
+  const int P = n + 2 * h;
+  const long np = static_cast<long>(P) * P * 6;
+  for (long g = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; g < np;
+       g += static_cast<long>(gridDim.x) * blockDim.x) {
+    const int i = static_cast<int>(g % P) - h;
+    const int j = static_cast<int>((g / P) % P) - h;
+    const bool corner = (i < 0 || i >= n) && (j < 0 || j >= n);
+    const bool interior = i >= 0 && i < n && j >= 0 && j < n;
+    if (corner) {
+      if (phi) phi[g] = 0.;
+      if (uc) uc[g] = 0.;
+      if (vc) vc[g] = 0.;
+      if (zonal) zonal[g] = 0.;
+      if (merid) merid[g] = 0.;
+      continue;
+    }
+    const double u = q.u[g], v = q.v[g];
+    if (phi) phi[g] = q.phiG[g] / st.sqrtG[g];
+    if (uc) uc[g] = st.IG11[g] * u + st.IG12[g] * v;
+    if (vc) vc[g] = st.IG21[g] * u + st.IG22[g] * v;
+    if (zonal) zonal[g] = interior ? st.M11[g] * u + st.M12[g] * v : 0.;
+    if (merid) merid[g] = interior ? st.M21[g] * u + st.M22[g] * v : 0.;
+  }
+}
+
+// Neumaier compensated accumulation.
+struct KSum {
+  double s, c;
+  __device__ void add(double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+    s = t;
+  }
+  __device__ void merge(const KSum& o) {
+    add(o.s);
+    c += o.c;
+  }
+};
+
+template <int NTH>
+__device__ KSum block_reduce(KSum v, double* sh_s, double* sh_c) {
+  sh_s[threadIdx.x] = v.s;
+  sh_c[threadIdx.x] = v.c;
+  __syncthreads();
+  for (int off = NTH / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      KSum a{sh_s[threadIdx.x], sh_c[threadIdx.x]};
+      a.merge(KSum{sh_s[threadIdx.x + off], sh_c[threadIdx.x + off]});
+      sh_s[threadIdx.x] = a.s;
+      sh_c[threadIdx.x] = a.c;
+    }
+    __syncthreads();
+  }
+  return KSum{sh_s[0], sh_c[0]};
+}
+
+constexpr int kDiagThreads = 256;
+
+__global__ void __launch_bounds__(kDiagThreads)
+    diag_partial_f64(Prog<const double> q, const double* phi0, const double* uc0, const double* vc0,
+                     StaticsF64 st, int n, int h, double* partials) {
+  __shared__ double sh_s[kDiagThreads], sh_c[kDiagThreads];
+  const int P = n + 2 * h;
+  const long ni = static_cast<long>(n) * n * 6;
+  KSum m{0., 0.}, e{0., 0.};
+  for (long c = blockIdx.x * static_cast<long>(kDiagThreads) + threadIdx.x; c < ni;
+       c += static_cast<long>(gridDim.x) * kDiagThreads) {
+    const int i = static_cast<int>(c % n), j = static_cast<int>((c / n) % n), p = static_cast<int>(c / (static_cast<long>(n) * n));
+    const long g = gidx(i, j, p, h, P);
+    const double phiG = q.phiG[g], u = q.u[g], v = q.v[g];
+    double phi, uc, vc;
+    if (phi0) {
+      phi = phi0[g]; uc = uc0[g]; vc = vc0[g];
+    } else {
+      phi = phiG / st.sqrtG[g];
+      uc = st.IG11[g] * u + st.IG12[g] * v;
+      vc = st.IG21[g] * u + st.IG22[g] * v;
+    }
+    // diag_mod.F90:20-26 and :40-52
+    const double KE = 0.5 * phi * (u * uc + v * vc);
+    const double pt = phi + st.phis[g];
+    const double PE = 0.5 * (pt * pt);
+    m.add(phiG);
+    e.add(st.sqrtG[g] * (KE + PE));
+  }
+  KSum rm = block_reduce<kDiagThreads>(m, sh_s, sh_c);
+  __syncthreads();
+  KSum re = block_reduce<kDiagThreads>(e, sh_s, sh_c);
+  if (threadIdx.x == 0) {
+    partials[4 * blockIdx.x + 0] = rm.s;
+    partials[4 * blockIdx.x + 1] = rm.c;
+    partials[4 * blockIdx.x + 2] = re.s;
+    partials[4 * blockIdx.x + 3] = re.c;
+  }
+}
+
+__global__ void __launch_bounds__(kDiagThreads) diag_final_f64(const double* partials, int nb, double* result) {
+  __shared__ double sh_s[kDiagThreads], sh_c[kDiagThreads];
+  KSum m{0., 0.}, e{0., 0.};
+  for (int b = threadIdx.x; b < nb; b += kDiagThreads) {
+    m.merge(KSum{partials[4 * b + 0], partials[4 * b + 1]});
+    e.merge(KSum{partials[4 * b + 2], partials[4 * b + 3]});
+  }
+  KSum rm = block_reduce<kDiagThreads>(m, sh_s, sh_c);
+  __syncthreads();
+  KSum re = block_reduce<kDiagThreads>(e, sh_s, sh_c);
+  if (threadIdx.x == 0) {
+    result[0] = rm.s + rm.c;
+    result[1] = re.s + re.c;
+  }
+}
+
+}  // namespace
+
+void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a, const StaticsF64& st,
+                      const HaloScatter& hs) {
+  const dim3 grid((a.n + TX - 1) / TX, (a.n + TY - 1) / TY, 6);
+  const dim3 block(NT);
+#define CCS_LAUNCH(K)                                                                   \
+  do {                                                                                 \
+    if (expl) hipLaunchKernelGGL((stage_f64<K, true>), grid, block, 0, s, a, st, hs);  \
+    else hipLaunchKernelGGL((stage_f64<K, false>), grid, block, 0, s, a, st, hs);      \
+  } while (0)
+  switch (kind) {
+    case kTendOnly: CCS_LAUNCH(kTendOnly); break;
+    case kRK3s1: CCS_LAUNCH(kRK3s1); break;
+    case kRK3s2: CCS_LAUNCH(kRK3s2); break;
+    case kRK3s3: CCS_LAUNCH(kRK3s3); break;
+    case kRK4a: CCS_LAUNCH(kRK4a); break;
+    default: CCS_LAUNCH(kRK4f); break;
+  }
+#undef CCS_LAUNCH
+}
+
+void launch_derived_f64(hipStream_t s, const Prog<const double>& q, const StaticsF64& st, int n, int h,
+                        double* phi, double* uc, double* vc, double* zonal, double* merid) {
+  hipLaunchKernelGGL(derived_f64, dim3(1024), dim3(256), 0, s, q, st, n, h, phi, uc, vc, zonal, merid);
+}
+
+void launch_diag_f64(hipStream_t s, const Prog<const double>& q, const double* phi, const double* uc,
+                     const double* vc, const StaticsF64& st, int n, int h, double* partials,
+                     double* result) {
+  hipLaunchKernelGGL(diag_partial_f64, dim3(kDiagBlocks), dim3(kDiagThreads), 0, s, q, phi, uc, vc, st, n,
+                     h, partials);
+  hipLaunchKernelGGL(diag_final_f64, dim3(1), dim3(kDiagThreads), 0, s, partials, kDiagBlocks, result);
+}
+
+}  // namespace ccs

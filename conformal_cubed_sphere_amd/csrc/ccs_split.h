@@ -1,0 +1,33 @@
+// ccs_split.h — split-precision (FP32 state, FP64-derived statics) path of
+// the time step, and its FP64 instantiation for throughput comparison.
+#ifndef CCS_SPLIT_H
+#define CCS_SPLIT_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "ccs.h"
+#include "ccs_kernels.h"
+
+namespace ccs {
+
+struct SplitState;
+
+int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out, std::string* err);
+void split_destroy(SplitState* s);
+// T: the 15 host-side SoA tables built by ccs_set_mesh (sqrtG, IG11, IG12,
+// IG21, IG22, phis, f, M11, M12, M21, M22, N11, N12, N21, N22), padded.
+int split_set_mesh(SplitState* s, hipStream_t st, double* const T[15], std::string* err);
+int split_set_state(SplitState* s, hipStream_t st, const double* phiG, const double* u, const double* v,
+                    const double* phi, const double* uc, const double* vc, std::string* err);
+int split_enqueue_step(SplitState* s, hipStream_t st, int scheme, bool first, hipEvent_t* ev);
+int split_get_state(SplitState* s, hipStream_t st, bool first, double* phiG, double* u, double* v,
+                    double* phi, double* uc, double* vc, double* zonal, double* merid, std::string* err);
+int split_tendency(SplitState* s, hipStream_t st, bool first, double* tp, double* tu, double* tv,
+                   std::string* err);
+int split_diagnostics(SplitState* s, hipStream_t st, bool first, double* result, std::string* err);
+
+}  // namespace ccs
+
+#endif

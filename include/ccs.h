@@ -1,0 +1,198 @@
+/*
+ * ccs.h — C ABI of the MI355X shallow-water dynamical core on the conformal
+ * cubed sphere (libccs.so).
+ *
+ * The reference (DwyaneChou/Conformal_Cubed_Sphere, dynamic_core/) is a
+ * Fortran program with no plugin or FFI layer: its operator interface is the
+ * module-procedure set called by the driver ccsm.F90.  Every entry point below
+ * names the reference procedure it replaces (file:line).  A Fortran host binds
+ * these through iso_c_binding (conformal_cubed_sphere_amd/fortran/ccs_gpu_mod.F90),
+ * Python through ctypes (conformal_cubed_sphere_amd/_lib.py).
+ *
+ * Conventions shared by all entry points
+ *  - Every int-returning function returns CCS_OK (0) or a ccs_status code.
+ *    Handle-based calls leave a message readable with ccs_last_error().
+ *  - Arrays are plain host pointers in the reference layout: Fortran order,
+ *    i fastest, then j, then panel.  "padded" arrays cover
+ *    i, j in 1-halo .. n+halo and panels 1..6, i.e. (n+2*halo)^2*6 values;
+ *    "interior" arrays cover i, j in 1..n, i.e. n*n*6 values.  2x2 matrices
+ *    are stored column-major per cell (element (r,c) at r + 2*c + 4*cell),
+ *    3x2 Jacobians likewise (r + 3*c + 6*cell).
+ *  - Halo corner cells (both i and j outside 1..n) are never filled and never
+ *    read, exactly as in the reference (ghost_mod.f90:34-76); the library
+ *    leaves them zero.
+ *  - A handle owns its device buffers and one HIP stream.  A handle is not
+ *    thread-safe; calls come from one host thread.
+ */
+#ifndef CCS_H
+#define CCS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum ccs_status {
+  CCS_OK = 0,
+  CCS_ERR_INVALID = 1,   /* bad argument: size, null pointer, unknown option   */
+  CCS_ERR_DEVICE = 2,    /* HIP runtime failure or no usable device            */
+  CCS_ERR_ORDER = 3,     /* call order violated (e.g. step before set_state)   */
+  CCS_ERR_NONFINITE = 4, /* NaN or Inf found in the prognostic state           */
+  CCS_ERR_NOMEM = 5,     /* host or device allocation failed                   */
+  CCS_ERR_SINGULAR = 6   /* singular metric matrix (reference io_mod.F90:177)   */
+};
+
+/* Arithmetic of the time-step path.
+ *  CCS_FP64_PARITY: FP64 with the reference's operation order kept operation
+ *    for operation (full 2x2 metric transforms, the covariant->sphere->covariant
+ *    round trip of correct_bdy_ghost, no FMA contraction).  This is the
+ *    drop-in path that matches the CPU reference to round-off.
+ *  CCS_FP32_SPLIT: FP32 state, fluxes and RK stages on a non-dimensional form
+ *    of the same equations; metric terms computed in FP64 once and cast; the
+ *    geopotential is stored as an FP32 deviation from an FP64 reference field;
+ *    diagnostics in FP64 with compensated sums.
+ *  CCS_FP64_FAST: the split-precision kernel instantiated in FP64 (same
+ *    algebraic shortcuts as FP32_SPLIT); a throughput comparison point. */
+enum ccs_precision { CCS_FP64_PARITY = 0, CCS_FP32_SPLIT = 1, CCS_FP64_FAST = 2 };
+
+/* Time scheme: reference namelist integral_scheme (parameters_mod.F90:129). */
+enum ccs_scheme { CCS_RK3_TVD = 0, CCS_RK4 = 1 };
+
+/* Rusanov dissipation of the Bernoulli-function flux.  The reference reads an
+ * uninitialised array there (spatial_operators_mod.F90:119,144), which is zero
+ * on fresh stack pages: CCS_EFLUX_NONE reproduces that reading. */
+enum ccs_eflux { CCS_EFLUX_NONE = 0 };
+
+/* ---------------------------------------------------------- mesh producer */
+/* Conformal cubed-sphere cell-centre geometry on the reference panel layout
+ * (mesh_generation/src/cuco.f90:100-105 frames, mesh_mod.f90:150-166 output):
+ * replaces the offline generator ccs.exe and its mesh file, which the dycore
+ * reads in io_mod.F90:37-117.  Interior arrays:
+ *   lon_deg, lat_deg : n*n*6 (degrees, atan2 ranges as cart2sph)
+ *   xyz              : 3*n*n*6 (unit sphere, x,y,z per cell)
+ *   jab              : 6*n*n*6 (d xyz / d(map angle), per radian, 3x2 per cell)
+ * Any output pointer may be NULL.  n must be even and >= 4. */
+int ccs_mesh_geometry(int n, double* lon_deg, double* lat_deg, double* xyz,
+                      double* jab);
+
+/* Static tables of the dycore on the padded grid, computed in FP64 exactly as
+ * io_mod.F90:110-304 does (halo fills of ghost_mod.f90:18-160, G = J^T J,
+ * Gauss-Jordan inverses of math_mod.f90:451-525, sqrtG = sqrt(det G),
+ * A = S(lon,lat) J, f = 2 Omega sin(lat), areaCell = sqrtG dx^2).
+ * Caller allocates every non-NULL array: scalars (n+2h)^2*6, matrices 4x that. */
+typedef struct ccs_metrics {
+  int n, halo;
+  double radius, omega;
+  double dx;                                 /* out: grid spacing, radians   */
+  double *lon, *lat;                         /* radians; corners zero         */
+  double *sinlon, *coslon, *sinlat, *coslat;
+  double *sqrtG, *f, *areaCell;
+  double *G, *IG, *A, *IA;                   /* 2x2 per cell                  */
+} ccs_metrics;
+
+int ccs_mesh_metrics(const double* lon_deg, const double* lat_deg,
+                     const double* jab, ccs_metrics* m);
+
+/* ---------------------------------------------------------- initial state */
+/* Initial conditions: Williamson cases 2, 5, 6 of test_case_mod.F90:61-204
+ * (case_num 2,5,6) and the seeded balanced jet of the benchmark recipe
+ * (case_num 7: u = u0 cos(lat), v = 0, depth in gradient-wind balance with the
+ * given area-mean depth, n_bumps Gaussian bumps and an optional Gaussian
+ * mountain, centres/amplitudes/radii from a SplitMix64 stream of `seed`).
+ * Fills padded arrays like initTestCase (test_case_mod.F90:13-58): phiG = phi
+ * sqrtG, covariant u,v = (G A^-1)(us,vs), contravariant uc,vc = A^-1 (us,vs).
+ * Any output pointer may be NULL except phiG, u, v, phi, uc, vc, phis. */
+typedef struct ccs_case_params {
+  int case_num;
+  int n_bumps;           /* case 7 */
+  int mountain;          /* case 7: 0 flat bottom, 1 Gaussian mountain */
+  int pad_;
+  unsigned long long seed;
+  double gravity;
+  double u0;             /* case 7 jet speed, m/s            (20)      */
+  double h_mean;         /* case 7 area-mean depth, m        (10000)   */
+  double bump_amp;       /* case 7 amplitudes U(-amp, amp), m (100)     */
+  double bump_rmin;      /* case 7 e-folding radii U(rmin, rmax), m    */
+  double bump_rmax;      /*                              (5e5, 1.5e6)  */
+  double mountain_h;     /* case 7 mountain height, m        (2000)    */
+  double mountain_r;     /* case 7 mountain e-folding radius (1.5e6)   */
+} ccs_case_params;
+
+void ccs_case_defaults(ccs_case_params* c);
+int ccs_init_case(const ccs_metrics* m, const ccs_case_params* c,
+                  double* phiG, double* u, double* v, double* phi, double* uc,
+                  double* vc, double* zonal, double* merid, double* phis);
+
+/* ------------------------------------------------------------- dycore handle */
+typedef struct ccs_handle ccs_handle;
+
+typedef struct ccs_config {
+  int n, halo;           /* halo >= 3 (WENO-5 and the 6th-order vorticity)    */
+  int precision;         /* enum ccs_precision                                */
+  int scheme;            /* enum ccs_scheme                                   */
+  int eflux;             /* enum ccs_eflux                                    */
+  int device;            /* HIP device ordinal                                */
+  int use_graph;         /* replay each step from a captured HIP graph        */
+  int pad_;
+  double dt, dx;         /* seconds; radians (ccs_metrics.dx)                 */
+  double radius, gravity, omega;
+} ccs_config;
+
+/* Allocate device state for one run (replaces initStat/initTend,
+ * stat_mod.F90:22-49, tend_mod.F90:17-28). */
+int ccs_create(const ccs_config* cfg, ccs_handle** out);
+
+/* Upload the static tables (the padded mesh%sqrtG, matrixG, matrixIG, matrixA,
+ * matrixIA, f, phis of mesh_mod.F90:9-48).  f is read on interior cells only. */
+int ccs_set_mesh(ccs_handle* h, const double* sqrtG, const double* G,
+                 const double* IG, const double* A, const double* IA,
+                 const double* f, const double* phis);
+
+/* Upload the prognostic state stat(old) (stat_mod.F90:7-17): padded phiG and
+ * covariant u, v.  phi, uc, vc are the diagnostic companions the reference
+ * carries; when given they are used by the first stage exactly as the
+ * reference's first spatial_operator call uses initTestCase's values, when
+ * NULL they are derived (phi = phiG/sqrtG, (uc,vc) = G^-1 (u,v)). */
+int ccs_set_state(ccs_handle* h, const double* phiG, const double* u,
+                  const double* v, const double* phi, const double* uc,
+                  const double* vc);
+
+/* Advance nsteps time steps: replaces the loop body
+ * `call RK3_TVD(stat(new), stat(old)); call copyStat(...)`
+ * (ccsm.F90:50-66, time_scheme_mod.F90:46-67) or RK4 (:13-44).
+ * Asynchronous on the handle's stream. */
+int ccs_step(ccs_handle* h, int nsteps);
+
+/* Copy the state back (padded).  zonal/merid are the sphere components on
+ * interior cells, as convert_wind_P2SP (spatial_operators_mod.F90:364-378)
+ * leaves them before a history write (ccsm.F90:58).  Any pointer may be NULL.
+ * Synchronises the stream. */
+int ccs_get_state(ccs_handle* h, double* phiG, double* u, double* v,
+                  double* phi, double* uc, double* vc, double* zonal,
+                  double* merid);
+
+/* Tendency L(q) of the current state on interior cells (n*n*6 each):
+ * spatial_operator (spatial_operators_mod.F90:13-181). */
+int ccs_tendency(ccs_handle* h, double* tend_phiG, double* tend_u,
+                 double* tend_v);
+
+/* Total mass sum(phiG) and total energy sum(sqrtG (KE+PE)) over interior
+ * cells (diag_mod.F90:10-53) in FP64 with compensated (Neumaier) sums. */
+int ccs_diagnostics(ccs_handle* h, double* total_mass, double* total_energy);
+
+/* Time nsteps steps with HIP events on the handle's stream.  total_ms: wall
+ * time of the whole loop; kernel_ms (optional): mean duration of one stage
+ * kernel measured with events around every stage launch in a second pass. */
+int ccs_time_steps(ccs_handle* h, int nsteps, double* total_ms,
+                   double* kernel_ms);
+
+int ccs_synchronize(ccs_handle* h);
+void ccs_destroy(ccs_handle* h);
+const char* ccs_last_error(const ccs_handle* h);
+const char* ccs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCS_H */
