@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Throughput of the dynamic-core time step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): C720
+(6 x 720^2 cells, halo 3), seeded balanced jet + 8 Gaussian bumps + Gaussian
+mountain, SSP-RK3 at CFL 0.5 (dt from the initial state over a 5-day run),
+a = 6.371e6 m, g = 9.80616, Omega = 7.292e-5.  A step is one SSP-RK3 step of
+all six panels (3 fused stage kernels).  Inputs are resident in HBM before the
+timed region.  The split-precision path (FP32 state) is the headline `value`;
+the FP64 parity path is timed beside it at the same configuration.
+
+Prints one JSON line.  `python bench.py --gpus N --steps K --warmup W`.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s at C720, FP32 vs FP64 GPU at matched L2(h) error; HBM GB/s"
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic bytes per cell-update (one interior cell, one full SSP-RK3 step):
+# 24 state words (3 stages x 3 read + 3 write, plus the base state q^n read in
+# stages 2 and 3) + 9 static words (3 stages x {sqrtG, f, phis}).
+BYTES_PER_CELL_UPDATE = {"f64": 24 * 8 + 9 * 8, "f32": 24 * 4 + 9 * 4}
+GRAVITY = 9.80616
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=720)
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def time_path(ccs, mesh, s0, dt, precision, steps, warmup):
+    core = ccs.DynamicalCore(mesh, dt, precision=precision, use_graph=True)
+    core.set_state(s0)
+    core.step(1)          # first step reads the uploaded companions
+    core.step(warmup)
+    core.synchronize()
+    t0 = time.perf_counter()
+    core.step(steps)
+    core.synchronize()
+    wall = time.perf_counter() - t0
+    ev_total, _ = core.time_steps(steps)            # HIP events around the graph replays
+    nk = min(steps, 60)
+    _, kern_ms = core.time_steps(nk, kernel_timing=True)  # events around every stage kernel
+    core.advanced = 1 + warmup + 2 * steps + 2 * nk  # time_steps(kernel_timing) runs two passes
+    return core, wall, ev_total, kern_ms
+
+
+def cpu_baseline(ccs, mesh, s0, dt, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib  # the checker, timed here as the CPU baseline
+
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+    orc = oracle_lib.Oracle(mesh, s0, dt, threads=cores)
+    t0 = time.perf_counter()
+    orc.steps(1)
+    one = time.perf_counter() - t0
+    k = max(1, min(50, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    orc.steps(k)
+    el = time.perf_counter() - t0
+    cells = 6 * mesh.n * mesh.n
+    return {"value": cells * k / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle/ccs_oracle.c (FP64 restatement of the reference step, OpenMP over rows) "
+                      f"on the same C{mesh.n} workload: {k} SSP-RK3 steps after 1 untimed step, "
+                      f"{el:.1f} s on {cores} host threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
+              "the multi-GPU panel sharding is a design in DESIGN.md, not a runnable path", file=sys.stderr)
+        sys.exit(2)
+    import conformal_cubed_sphere_amd as ccs
+
+    mesh = ccs.Mesh(args.n, halo=3, radius=6.371e6, omega=7.292e-5)
+    s0 = ccs.init_case(mesh, case_num=7, seed=1, n_bumps=8, mountain=True, gravity=GRAVITY)
+    dt, nsteps_5d = ccs.cfl_time_step(mesh, s0, 0.5, 5 * 86400.0)
+    cells = 6 * args.n * args.n
+
+    headline = args.precision
+    if headline == "auto":
+        headline = "fp32"
+        try:
+            ccs.DynamicalCore(mesh, dt, precision="fp32").close()
+        except ccs.CcsError:
+            headline = "fp64"
+    dtype = "f32" if headline == "fp32" else "f64"
+    core, wall, ev_total, kern_ms = time_path(ccs, mesh, s0, dt, headline, args.steps, args.warmup)
+    value = cells * args.steps / wall
+    bpc = BYTES_PER_CELL_UPDATE[dtype]
+    stages = 3
+    achieved = bpc * cells / stages / (kern_ms * 1e-3) / 1e9  # GB/s of the stage kernel
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+        "config": {"workload": f"C{args.n} seeded balanced jet + 8 bumps + Gaussian mountain, SSP-RK3, "
+                               f"CFL 0.5, halo 3, precision {headline}",
+                   "n": args.n, "halo": 3, "dt_s": dt, "steps_per_5_days": nsteps_5d,
+                   "interior_cells": cells, "precision": headline},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "stage kernel (fused tendency + RK stage + halo exchange)",
+                     "kernel_ms_avg": kern_ms, "algorithmic_bytes_per_cell_update": bpc},
+        "event_ms_per_step": ev_total / args.steps,
+    }
+    prof = os.path.join(ROOT, "profiles", f"traffic_c{args.n}_{headline}.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            t = json.load(f)
+        out["roofline"]["traffic"] = t.get("hbm_bytes_per_launch")
+        out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
+
+    if headline == "fp32" and not args.no_fp64:
+        c64, wall64, _, kern64 = time_path(ccs, mesh, s0, dt, "fp64", max(20, args.steps // 3), 5)
+        v64 = cells * max(20, args.steps // 3) / wall64
+        out["fp64_parity_path"] = {"value": v64, "unit": "cell-updates/s", "kernel_ms_avg": kern64,
+                                   "hbm_gbps_algorithmic": BYTES_PER_CELL_UPDATE["f64"] * cells / 3 / (kern64 * 1e-3) / 1e9}
+        out["fp32_over_fp64"] = value / v64
+        # matched-error check on the same number of steps from the same start
+        s32 = core.get_state()
+        core64 = ccs.DynamicalCore(mesh, dt, precision="fp64")
+        core64.set_state(s0)
+        core64.step(core.advanced)
+        s64 = core64.get_state()
+        core.close(); c64.close(); core64.close()
+        area = mesh.interior(mesh.areaCell)
+        h32, h64 = mesh.interior(s32.phi) / GRAVITY, mesh.interior(s64.phi) / GRAVITY
+        out["l2_h_fp32_vs_fp64"] = ccs.error_norms(h32, h64, area)[1]
+        out["l2_h_compared_after_steps"] = core.advanced
+    else:
+        core.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ccs, mesh, s0, dt, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

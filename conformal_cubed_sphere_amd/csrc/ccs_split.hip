@@ -1,22 +1,729 @@
-// ccs_split.hip — split-precision path (placeholder until implemented).
+// ccs_split.hip — split-precision time step (FP32 state, metric tables
+// computed in FP64 once and cast) and its FP64 instantiation.
+//
+// Same equations and the same WENO-Z / Rusanov / 6th-order-vorticity method as
+// the reference spatial operator (spatial_operators_mod.F90:13-181), written
+// in a non-dimensional form that keeps FP32 in range:
+//   P = phiG / a^2   (= phi * sqrtg, sqrtg = sqrtG / a^2 the unit-sphere Jacobian)
+//   U = u / a, V = v / a   (covariant winds), tau = t / a.
+// On a conformal grid G = sqrtG I, so the contravariant winds are U/sqrtg,
+// V/sqrtg, G^-1_11 = G^-1_22 = 1/sqrtG, and the covariant->sphere->covariant
+// round trip of correct_bdy_ghost is the identity.  The equations become
+//   dP/dtau = -(dFx/dx + dFy/dy),        Fx = P Uc, Fy = P Vc
+//   dU/dtau = -dE/dx + zeta Vc,          E = phi + phis + (U Uc + V Vc)/2
+//   dV/dtau = -dE/dy - zeta Uc,          zeta = dV/dx - dU/dy + a sqrtg f
+// and the Rusanov speed is |Ub| + sqrt(gb pb) with gb the face mean of
+// 1/sqrtg.  The WENO-Z epsilon scales with the square of each variable's
+// scale so the nonlinear weights equal the reference's: 1e-2/a^2 for Fx, Fy,
+// 1e-2 for E, 1e-2/a^4 for phitG/a^2.
+//
+// The geopotential is carried as an FP32 deviation dP from an FP64 reference
+// field Pref (the state at the last upload); the RK stage combinations have
+// weights summing to one, so they act on dP alone and the FP32 increments are
+// not lost against the ~1e5 magnitude of P.
+//
+// Kernel structure (one launch per RK stage, as the FP64 path):
+//   phase 1  pointwise terms on the tile and a 3-cell cross border -> LDS
+//   phase 2a one task per (window centre, variable): the smoothness
+//            indicators of the 5-point window are computed once and used by
+//            both reconstructions the window feeds (the + value at its right
+//            face and the - value at its left face; the reference recomputes
+//            them per face, spatial_operators_mod.F90:80-99)
+//   phase 2b Rusanov flux per face
+//   phase 3  divergence, vorticity, RK update, halo scatter with the signed
+//            permutation of the panel-basis change (ghost_mod.f90:100-153).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ccs_internal.h"
 #include "ccs_split.h"
 
 namespace ccs {
+namespace {
 
-struct SplitState {};
+constexpr int TX = 32;
+constexpr int TY = 8;
+constexpr int NT = TX * TY;
+constexpr int EX = TX + 6;
+constexpr int EY = TY + 6;
+constexpr int NE = EX * EY;
 
-int split_create(const ccs_config&, const HaloScatter&, SplitState**, std::string* err) {
-  *err = "split-precision path not available in this build";
-  return CCS_ERR_INVALID;
+__device__ __forceinline__ long gidx(int i, int j, int p, int h, int P) {
+  return static_cast<long>(i + h) + static_cast<long>(P) * (static_cast<long>(j + h) + static_cast<long>(P) * p);
 }
-void split_destroy(SplitState* s) { delete s; }
-int split_set_mesh(SplitState*, hipStream_t, double* const*, std::string*) { return CCS_ERR_INVALID; }
-int split_set_state(SplitState*, hipStream_t, const double*, const double*, const double*, const double*,
-                    const double*, const double*, std::string*) { return CCS_ERR_INVALID; }
-int split_enqueue_step(SplitState*, hipStream_t, int, bool, hipEvent_t*) { return CCS_ERR_INVALID; }
-int split_get_state(SplitState*, hipStream_t, bool, double*, double*, double*, double*, double*, double*,
-                    double*, double*, std::string*) { return CCS_ERR_INVALID; }
-int split_tendency(SplitState*, hipStream_t, bool, double*, double*, double*, std::string*) { return CCS_ERR_INVALID; }
-int split_diagnostics(SplitState*, hipStream_t, bool, double*, std::string*) { return CCS_ERR_INVALID; }
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
+template <typename T>
+struct SplitStatics {
+  const T* pref;  // FP32 copy of the reference P (padded)
+  const T* sg;    // sqrtG / a^2 (padded)
+  const T* phis;  // surface geopotential (padded)
+  const T* fs;    // a * sqrtg * f (interior cells of the padded layout)
+};
+
+template <typename T>
+struct SplitArgs {
+  Prog<const T> in, base;
+  Prog<T> out;
+  T* tend[3];
+  const T* tacc[9];
+  int n, h;
+  T rdx;               // 1 / dx
+  T dts, incs;         // dt / a, RK4 increment / a
+  T epsF, epsE, epsP;  // WENO-Z epsilons of Fx/Fy, E, phitG/a^2
+};
+
+// One 5-point window (q0..q4, centre q2): the WENO-Z value at the centre
+// cell's right face (dir > 0 of WENO_limiter) and at its left face (dir < 0),
+// sharing the smoothness indicators.  Weight rule of
+// spatial_operators_mod.F90:348-355; for the mirrored (left-face) window the
+// indicators reverse order, which swaps tau40 and tau41.
+template <typename T>
+__device__ __forceinline__ void weno_pair(T q0, T q1, T q2, T q3, T q4, T eps, T& plus, T& minus) {
+#pragma clang fp contract(fast)
+  const T a1 = q0 - T(2) * q1 + q2;
+  const T a2 = q1 - T(2) * q2 + q3;
+  const T a3 = q2 - T(2) * q3 + q4;
+  const T b1 = q0 - T(4) * q1 + T(3) * q2;
+  const T b2 = q1 - q3;
+  const T b3 = T(3) * q2 - T(4) * q3 + q4;
+  const T c13 = T(13) / T(12);
+  const T be1 = c13 * a1 * a1 + T(0.25) * b1 * b1;
+  const T be2 = c13 * a2 * a2 + T(0.25) * b2 * b2;
+  const T be3 = c13 * a3 * a3 + T(0.25) * b3 * b3;
+  const T t40 = fabs(be1 - be2);
+  const T t41 = fabs(be2 - be3);
+  const T t5 = fabs(be3 - be1);
+  const T mb = fmin(be1, fmin(be2, be3));
+  const T big = T(1e30);
+  const T tr1 = fmin(t5 * rcp(eps + be1), big);
+  const T tr2 = fmin(t5 * rcp(eps + be2), big);
+  const T tr3 = fmin(t5 * rcp(eps + be3), big);
+  const T s6 = T(1) / T(6);
+  {
+    const T s1 = (T(2) * q0 - T(7) * q1 + T(11) * q2) * s6;
+    const T s2 = (-q1 + T(5) * q2 + T(2) * q3) * s6;
+    const T s3 = (T(2) * q2 + T(5) * q3 - q4) * s6;
+    const T al1 = T(0.1) + T(0.1) * tr1, al2 = T(0.6) + T(0.6) * tr2, al3 = T(0.3) + T(0.3) * tr3;
+    const T gen = (al1 * s1 + al2 * s2 + al3 * s3) * rcp(al1 + al2 + al3);
+    const T br1 = (s1 + T(2) * s2) * (T(1) / T(3));
+    const T br2 = (T(2) * s2 + s3) * (T(1) / T(3));
+    plus = (t40 <= mb && t41 > mb) ? br1 : ((t40 > mb && t41 < mb) ? br2 : gen);
+  }
+  {
+    const T s1 = (T(2) * q4 - T(7) * q3 + T(11) * q2) * s6;
+    const T s2 = (-q3 + T(5) * q2 + T(2) * q1) * s6;
+    const T s3 = (T(2) * q2 + T(5) * q1 - q0) * s6;
+    const T al1 = T(0.1) + T(0.1) * tr3, al2 = T(0.6) + T(0.6) * tr2, al3 = T(0.3) + T(0.3) * tr1;
+    const T gen = (al1 * s1 + al2 * s2 + al3 * s3) * rcp(al1 + al2 + al3);
+    const T br1 = (s1 + T(2) * s2) * (T(1) / T(3));
+    const T br2 = (T(2) * s2 + s3) * (T(1) / T(3));
+    minus = (t41 <= mb && t40 > mb) ? br1 : ((t41 > mb && t40 < mb) ? br2 : gen);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T d6f(T m3, T m2, T m1, T p1, T p2, T p3, T rdx) {
+#pragma clang fp contract(fast)
+  return ((p3 - m3) - T(9) * (p2 - m2) + T(45) * (p1 - m1)) * (rdx * (T(1) / T(60)));
+}
+
+template <typename T, int KIND>
+__global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
+#pragma clang fp contract(fast)
+  __shared__ T sFx[NE], sFy[NE], sE[NE], sPt[NE], sUC[NE], sVC[NE], sPh[NE], sRg[NE], sU[NE], sV[NE];
+  constexpr int NFXF = TY * (TX + 1);
+  constexpr int NFYF = (TY + 1) * TX;
+  __shared__ T xp[3][NFXF], xm[3][NFXF], yp[3][NFYF], ym[3][NFYF];
+
+  const int n = a.n, h = a.h, P = n + 2 * h;
+  const int p = blockIdx.z;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int tid = threadIdx.x;
+
+  // ---- phase 1
+  constexpr int NX = EX * TY;
+  constexpr int NCROSS = NX + 6 * TX;
+  for (int e = tid; e < NCROSS; e += NT) {
+    int li, lj;
+    if (e < NX) {
+      li = e % EX; lj = 3 + e / EX;
+    } else {
+      const int e2 = e - NX;
+      li = 3 + e2 % TX;
+      const int r = e2 / TX;
+      lj = (r < 3) ? r : TY + r;
+    }
+    const int i = i0 + li - 3, j = j0 + lj - 3;
+    if (i < -h || i >= n + h || j < -h || j >= n + h) continue;
+    if ((i < 0 || i >= n) && (j < 0 || j >= n)) continue;
+    const long g = gidx(i, j, p, h, P);
+    const T Pv = st.pref[g] + a.in.phiG[g];
+    const T U = a.in.u[g], V = a.in.v[g];
+    const T sg = st.sg[g];
+    const T rg = rcp(sg);
+    const T Uc = U * rg, Vc = V * rg;
+    const T phit = Pv * rg + st.phis[g];
+    const int l = lj * EX + li;
+    sE[l] = phit + T(0.5) * (U * Uc + V * Vc);
+    sFx[l] = Pv * Uc;
+    sFy[l] = Pv * Vc;
+    sPt[l] = phit * sg;
+    sPh[l] = phit;
+    sRg[l] = rg;
+    sUC[l] = Uc;
+    sVC[l] = Vc;
+    sU[l] = U;
+    sV[l] = V;
+  }
+  __syncthreads();
+
+  // ---- phase 2a: one task per (window centre, variable)
+  constexpr int NWX = (TX + 2) * TY;
+  constexpr int NWY = TX * (TY + 2);
+  for (int e = tid; e < 3 * (NWX + NWY); e += NT) {
+    if (e < 3 * NWX) {
+      const int var = e / NWX, w = e % NWX;
+      const int ty = w / (TX + 2), c = w % (TX + 2) - 1;  // centre cell, tile-local
+      const int ig = i0 + c, j = j0 + ty;
+      if (j >= n || ig < -1 || ig > n) continue;
+      const T* s = (var == 0) ? sFx : ((var == 1) ? sE : sPt);
+      const T eps = (var == 0) ? a.epsF : ((var == 1) ? a.epsE : a.epsP);
+      const int l = (ty + 3) * EX + c + 3;
+      T pl, mi;
+      weno_pair(s[l - 2], s[l - 1], s[l], s[l + 1], s[l + 2], eps, pl, mi);
+      if (c + 1 <= TX) xp[var][ty * (TX + 1) + c + 1] = pl;  // right face of c
+      if (c >= 0) xm[var][ty * (TX + 1) + c] = mi;             // left face of c
+    } else {
+      const int e2 = e - 3 * NWX;
+      const int var = e2 / NWY, w = e2 % NWY;
+      const int tx = w % TX, c = w / TX - 1;
+      const int i = i0 + tx, jg = j0 + c;
+      if (i >= n || jg < -1 || jg > n) continue;
+      const T* s = (var == 0) ? sFy : ((var == 1) ? sE : sPt);
+      const T eps = (var == 0) ? a.epsF : ((var == 1) ? a.epsE : a.epsP);
+      const int l = (c + 3) * EX + tx + 3;
+      T pl, mi;
+      weno_pair(s[l - 2 * EX], s[l - EX], s[l], s[l + EX], s[l + 2 * EX], eps, pl, mi);
+      if (c + 1 <= TY) yp[var][(c + 1) * TX + tx] = pl;
+      if (c >= 0) ym[var][c * TX + tx] = mi;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2b: Rusanov fluxes (results overwrite xp[0]/xp[1], yp[0]/yp[1])
+  for (int e = tid; e < NFXF + NFYF; e += NT) {
+    if (e < NFXF) {
+      const int fi = e % (TX + 1), ty = e / (TX + 1);
+      const int ig = i0 + fi - 1, j = j0 + ty;
+      if (j >= n || ig < -1 || ig > n - 1) continue;
+      const int l = (ty + 3) * EX + fi + 2;  // cell ig; face between l and l+1
+      const T ub = T(0.5) * (sUC[l] + sUC[l + 1]);
+      const T pb = T(0.5) * (sPh[l] + sPh[l + 1]);
+      const T gb = T(0.5) * (sRg[l] + sRg[l + 1]);
+      const T lam = fabs(ub) + sqrt(gb * pb);
+      const int f = e;
+      const T F = T(0.5) * (xp[0][f] + xm[0][f] - lam * (xm[2][f] - xp[2][f]));
+      const T E = T(0.5) * (xp[1][f] + xm[1][f]);
+      xp[0][f] = F;
+      xp[1][f] = E;
+    } else {
+      const int f = e - NFXF;
+      const int tx = f % TX, fj = f / TX;
+      const int i = i0 + tx, jg = j0 + fj - 1;
+      if (i >= n || jg < -1 || jg > n - 1) continue;
+      const int l = (fj + 2) * EX + tx + 3;
+      const T vb = T(0.5) * (sVC[l] + sVC[l + EX]);
+      const T pb = T(0.5) * (sPh[l] + sPh[l + EX]);
+      const T gb = T(0.5) * (sRg[l] + sRg[l + EX]);
+      const T lam = fabs(vb) + sqrt(gb * pb);
+      const T F = T(0.5) * (yp[0][f] + ym[0][f] - lam * (ym[2][f] - yp[2][f]));
+      const T E = T(0.5) * (yp[1][f] + ym[1][f]);
+      yp[0][f] = F;
+      yp[1][f] = E;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 3: cells
+  const int tx = tid % TX, ty = tid / TX;
+  const int i = i0 + tx, j = j0 + ty;
+  if (i >= n || j >= n) return;
+  const T rdx = a.rdx;
+  const int lc = (ty + 3) * EX + (tx + 3);
+  const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
+  const T dFx = (xp[0][fx + 1] - xp[0][fx]) * rdx;
+  const T dEx = (xp[1][fx + 1] - xp[1][fx]) * rdx;
+  const T dFy = (yp[0][fy + TX] - yp[0][fy]) * rdx;
+  const T dEy = (yp[1][fy + TX] - yp[1][fy]) * rdx;
+  const T dvdx = d6f(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], rdx);
+  const T dudy = d6f(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX], sU[lc + 3 * EX], rdx);
+  const long g = gidx(i, j, p, h, P);
+  const T zeta = dvdx - dudy + st.fs[g];
+  const T tp = -(dFx + dFy);
+  const T tu = -dEx + zeta * sVC[lc];
+  const T tv = -dEy - zeta * sUC[lc];
+  const long ii = static_cast<long>(i) + static_cast<long>(n) * (static_cast<long>(j) + static_cast<long>(n) * p);
+  if (KIND == kTendOnly) {
+    a.tend[0][ii] = tp;
+    a.tend[1][ii] = tu;
+    a.tend[2][ii] = tv;
+    return;
+  }
+  T pn, un, vn;
+  if (KIND == kRK3s1) {
+    pn = a.base.phiG[g] + a.dts * tp;
+    un = a.base.u[g] + a.dts * tu;
+    vn = a.base.v[g] + a.dts * tv;
+  } else if (KIND == kRK3s2) {
+    const T w = T(0.25) * a.dts;
+    pn = T(0.75) * a.base.phiG[g] + T(0.25) * a.in.phiG[g] + w * tp;
+    un = T(0.75) * a.base.u[g] + T(0.25) * a.in.u[g] + w * tu;
+    vn = T(0.75) * a.base.v[g] + T(0.25) * a.in.v[g] + w * tv;
+  } else if (KIND == kRK3s3) {
+    const T c3 = T(1) / T(3), c23 = T(2) / T(3);
+    const T w = c23 * a.dts;
+    pn = c3 * a.base.phiG[g] + c23 * a.in.phiG[g] + w * tp;
+    un = c3 * a.base.u[g] + c23 * a.in.u[g] + w * tu;
+    vn = c3 * a.base.v[g] + c23 * a.in.v[g] + w * tv;
+  } else if (KIND == kRK4a) {
+    a.tend[0][ii] = tp;
+    a.tend[1][ii] = tu;
+    a.tend[2][ii] = tv;
+    pn = a.base.phiG[g] + a.incs * tp;
+    un = a.base.u[g] + a.incs * tu;
+    vn = a.base.v[g] + a.incs * tv;
+  } else {
+    const T c6 = T(1) / T(6);
+    pn = a.base.phiG[g] + a.dts * ((a.tacc[0][ii] + T(2) * (a.tacc[3][ii] + a.tacc[6][ii]) + tp) * c6);
+    un = a.base.u[g] + a.dts * ((a.tacc[1][ii] + T(2) * (a.tacc[4][ii] + a.tacc[7][ii]) + tu) * c6);
+    vn = a.base.v[g] + a.dts * ((a.tacc[2][ii] + T(2) * (a.tacc[5][ii] + a.tacc[8][ii]) + tv) * c6);
+  }
+  a.out.phiG[g] = pn;
+  a.out.u[g] = un;
+  a.out.v[g] = vn;
+  const int i1 = i + 1, j1 = j + 1;
+  const bool strip[4] = {i1 <= h, i1 >= n - h + 1, j1 <= h, j1 >= n - h + 1};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (!strip[s]) continue;
+    const int* r = hs.r[p][s];
+    const int id = r[1] + r[2] * i1 + r[3] * j1;
+    const int jd = r[4] + r[5] * i1 + r[6] * j1;
+    const long gd = gidx(id - 1, jd - 1, r[0] - 1, h, P);
+    const T c[2] = {un, vn};
+    a.out.phiG[gd] = pn;
+    a.out.u[gd] = static_cast<T>(r[8]) * c[r[7]];
+    a.out.v[gd] = static_cast<T>(r[10]) * c[r[9]];
+  }
+}
+
+// Compensated FP64 mass/energy of the split state.
+struct KSum {
+  double s, c;
+  __device__ void add(double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+    s = t;
+  }
+};
+
+constexpr int kDT = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(kDT) split_diag(Prog<const T> q, const double* pref64, const double* sqrtG,
+                                                  const double* phis, int n, int h, double a, double* partials) {
+  __shared__ double sh[4][kDT];
+  const int P = n + 2 * h;
+  const long ni = static_cast<long>(n) * n * 6;
+  KSum m{0., 0.}, e{0., 0.};
+  const double a2 = a * a;
+  for (long c = blockIdx.x * static_cast<long>(kDT) + threadIdx.x; c < ni; c += static_cast<long>(gridDim.x) * kDT) {
+    const int i = static_cast<int>(c % n), j = static_cast<int>((c / n) % n), p = static_cast<int>(c / (static_cast<long>(n) * n));
+    const long g = gidx(i, j, p, h, P);
+    const double phiG = a2 * (pref64[g] + static_cast<double>(q.phiG[g]));
+    const double u = a * static_cast<double>(q.u[g]), v = a * static_cast<double>(q.v[g]);
+    const double sG = sqrtG[g];
+    const double phi = phiG / sG;
+    const double KE = 0.5 * phi * ((u * u + v * v) / sG);
+    const double pt = phi + phis[g];
+    m.add(phiG);
+    e.add(sG * (KE + 0.5 * (pt * pt)));
+  }
+  sh[0][threadIdx.x] = m.s; sh[1][threadIdx.x] = m.c; sh[2][threadIdx.x] = e.s; sh[3][threadIdx.x] = e.c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    KSum M{0., 0.}, Ee{0., 0.};
+    double cm = 0., ce = 0.;
+    for (int t = 0; t < kDT; ++t) {
+      M.add(sh[0][t]); cm += sh[1][t];
+      Ee.add(sh[2][t]); ce += sh[3][t];
+    }
+    partials[4 * blockIdx.x + 0] = M.s;
+    partials[4 * blockIdx.x + 1] = M.c + cm;
+    partials[4 * blockIdx.x + 2] = Ee.s;
+    partials[4 * blockIdx.x + 3] = Ee.c + ce;
+  }
+}
+
+__global__ void split_diag_final(const double* partials, int nb, double* result) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  KSum M{0., 0.}, E{0., 0.};
+  double cm = 0., ce = 0.;
+  for (int b = 0; b < nb; ++b) {
+    M.add(partials[4 * b + 0]); cm += partials[4 * b + 1];
+    E.add(partials[4 * b + 2]); ce += partials[4 * b + 3];
+  }
+  result[0] = M.s + (M.c + cm);
+  result[1] = E.s + (E.c + ce);
+}
+
+template <typename T>
+void launch_split(hipStream_t s, int kind, const SplitArgs<T>& a, const SplitStatics<T>& st, const HaloScatter& hs) {
+  const dim3 grid((a.n + TX - 1) / TX, (a.n + TY - 1) / TY, 6);
+  const dim3 block(NT);
+  switch (kind) {
+    case kTendOnly: hipLaunchKernelGGL((stage_split<T, kTendOnly>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_split<T, kRK3s1>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_split<T, kRK3s2>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_split<T, kRK3s3>), grid, block, 0, s, a, st, hs); break;
+    case kRK4a: hipLaunchKernelGGL((stage_split<T, kRK4a>), grid, block, 0, s, a, st, hs); break;
+    default: hipLaunchKernelGGL((stage_split<T, kRK4f>), grid, block, 0, s, a, st, hs); break;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+struct SplitState {
+  ccs_config cfg{};
+  HaloScatter hs{};
+  bool fp64 = false;
+  int n = 0, h = 0;
+  std::size_t np = 0, ni = 0, esz = 4;
+  void* dmem = nullptr;
+  // typed device arrays (float or double)
+  void *pref = nullptr, *sg = nullptr, *phis = nullptr, *fs = nullptr;
+  void* buf[3][3]{};
+  void* tend[9]{};
+  double *pref64_d = nullptr, *sqrtG_d = nullptr, *phis_d = nullptr, *partials = nullptr, *result = nullptr;
+  // host FP64 tables for output
+  std::vector<double> sqrtG, IG11, IG12, IG21, IG22, M11, M12, M21, M22, pref64;
+  bool mesh_set = false;
+};
+
+namespace {
+
+constexpr int kSplitDiagBlocks = 256;
+
+template <typename T>
+SplitStatics<T> statics_of(const SplitState* s) {
+  return SplitStatics<T>{static_cast<const T*>(s->pref), static_cast<const T*>(s->sg),
+                         static_cast<const T*>(s->phis), static_cast<const T*>(s->fs)};
+}
+
+template <typename T>
+Prog<T> prog_of(const SplitState* s, int b) {
+  return Prog<T>{static_cast<T*>(s->buf[b][0]), static_cast<T*>(s->buf[b][1]), static_cast<T*>(s->buf[b][2])};
+}
+
+template <typename T>
+Prog<const T> cprog_of(const SplitState* s, int b) {
+  return Prog<const T>{static_cast<const T*>(s->buf[b][0]), static_cast<const T*>(s->buf[b][1]),
+                       static_cast<const T*>(s->buf[b][2])};
+}
+
+template <typename T>
+SplitArgs<T> base_args(const SplitState* s) {
+  SplitArgs<T> a{};
+  const double r = s->cfg.radius;
+  a.n = s->n;
+  a.h = s->h;
+  a.rdx = static_cast<T>(1.0 / s->cfg.dx);
+  a.dts = static_cast<T>(s->cfg.dt / r);
+  a.epsF = static_cast<T>(1e-2 / (r * r));
+  a.epsE = static_cast<T>(1e-2);
+  a.epsP = static_cast<T>(1e-2 / (r * r * r * r));
+  return a;
+}
+
+template <typename T>
+void enqueue_step_t(SplitState* s, hipStream_t st, int scheme, hipEvent_t* ev) {
+  SplitArgs<T> a = base_args<T>(s);
+  const SplitStatics<T> S = statics_of<T>(s);
+  int stage = 0;
+  auto launch = [&](int kind) {
+    if (ev) (void)hipEventRecord(ev[2 * stage], st);
+    launch_split<T>(st, kind, a, S, s->hs);
+    if (ev) (void)hipEventRecord(ev[2 * stage + 1], st);
+    ++stage;
+  };
+  if (scheme == CCS_RK3_TVD) {
+    a.in = cprog_of<T>(s, 0); a.base = cprog_of<T>(s, 0); a.out = prog_of<T>(s, 1);
+    launch(kRK3s1);
+    a.in = cprog_of<T>(s, 1); a.out = prog_of<T>(s, 2);
+    launch(kRK3s2);
+    a.in = cprog_of<T>(s, 2); a.out = prog_of<T>(s, 0);
+    launch(kRK3s3);
+  } else {
+    const double r = s->cfg.radius;
+    a.base = cprog_of<T>(s, 0);
+    a.in = cprog_of<T>(s, 0); a.out = prog_of<T>(s, 1); a.incs = static_cast<T>(0.5 * s->cfg.dt / r);
+    for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<T*>(s->tend[k]);
+    launch(kRK4a);
+    a.in = cprog_of<T>(s, 1); a.out = prog_of<T>(s, 2);
+    for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<T*>(s->tend[3 + k]);
+    launch(kRK4a);
+    a.in = cprog_of<T>(s, 2); a.out = prog_of<T>(s, 1); a.incs = static_cast<T>(s->cfg.dt / r);
+    for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<T*>(s->tend[6 + k]);
+    launch(kRK4a);
+    a.in = cprog_of<T>(s, 1); a.out = prog_of<T>(s, 0);
+    for (int k = 0; k < 9; ++k) a.tacc[k] = static_cast<const T*>(s->tend[k]);
+    launch(kRK4f);
+  }
+}
+
+template <typename T>
+void upload_cast(hipStream_t st, void* dst, const std::vector<double>& src, std::vector<char>& staging) {
+  staging.resize(src.size() * sizeof(T));
+  T* t = reinterpret_cast<T*>(staging.data());
+  for (std::size_t k = 0; k < src.size(); ++k) t[k] = static_cast<T>(src[k]);
+  (void)hipMemcpyAsync(dst, staging.data(), staging.size(), hipMemcpyHostToDevice, st);
+  (void)hipStreamSynchronize(st);
+}
+
+template <typename T>
+void download_to_double(hipStream_t st, const void* src, std::size_t cnt, std::vector<double>& out) {
+  std::vector<T> tmp(cnt);
+  (void)hipMemcpyAsync(tmp.data(), src, cnt * sizeof(T), hipMemcpyDeviceToHost, st);
+  (void)hipStreamSynchronize(st);
+  out.resize(cnt);
+  for (std::size_t k = 0; k < cnt; ++k) out[k] = static_cast<double>(tmp[k]);
+}
+
+}  // namespace
+
+int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out, std::string* err) {
+  SplitState* s = new SplitState();
+  s->cfg = cfg;
+  s->hs = hs;
+  s->fp64 = (cfg.precision == CCS_FP64_FAST);
+  s->esz = s->fp64 ? 8 : 4;
+  s->n = cfg.n;
+  s->h = cfg.halo;
+  s->np = padded_cells(cfg.n, cfg.halo);
+  s->ni = static_cast<std::size_t>(cfg.n) * cfg.n * 6;
+  // typed: 4 statics + 9 state (padded) + 9 tendencies (interior)
+  const std::size_t typed = (4 + 9) * s->np + 9 * s->ni;
+  const std::size_t dbl = 3 * s->np + 4 * kSplitDiagBlocks + 2;
+  const std::size_t bytes = typed * s->esz + dbl * sizeof(double) + 256;
+  if (hipMalloc(&s->dmem, bytes) != hipSuccess) {
+    *err = "split_create: device allocation failed";
+    delete s;
+    return CCS_ERR_NOMEM;
+  }
+  if (hipMemset(s->dmem, 0, bytes) != hipSuccess) {
+    *err = "split_create: hipMemset failed";
+    (void)hipFree(s->dmem);
+    delete s;
+    return CCS_ERR_DEVICE;
+  }
+  char* p = static_cast<char*>(s->dmem);
+  auto take = [&p](std::size_t b) { void* r = p; p += (b + 255) / 256 * 256; return r; };
+  double* d = reinterpret_cast<double*>(take(dbl * sizeof(double)));
+  s->pref64_d = d; s->sqrtG_d = d + s->np; s->phis_d = d + 2 * s->np;
+  s->partials = d + 3 * s->np; s->result = s->partials + 4 * kSplitDiagBlocks;
+  s->pref = take(s->np * s->esz); s->sg = take(s->np * s->esz); s->phis = take(s->np * s->esz); s->fs = take(s->np * s->esz);
+  for (int b = 0; b < 3; ++b)
+    for (int k = 0; k < 3; ++k) s->buf[b][k] = take(s->np * s->esz);
+  for (int k = 0; k < 9; ++k) s->tend[k] = take(s->ni * s->esz);
+  *out = s;
+  return CCS_OK;
+}
+
+void split_destroy(SplitState* s) {
+  if (!s) return;
+  if (s->dmem) (void)hipFree(s->dmem);
+  delete s;
+}
+
+int split_set_mesh(SplitState* s, hipStream_t st, double* const T[15], std::string* err) {
+  const int n = s->n, h = s->h;
+  const std::size_t np = s->np;
+  const double r = s->cfg.radius;
+  // The split path uses G = sqrtG I: reject grids that are not conformal.
+  for (int p = 1; p <= 6; ++p)
+    for (int j = 1 - h; j <= n + h; ++j)
+      for (int i = 1 - h; i <= n + h; ++i) {
+        if (is_corner(n, i, j)) continue;
+        const std::size_t d = pidx(n, h, i, j, p);
+        const double g = T[0][d];
+        const double off = std::fabs(T[2][d]) + std::fabs(T[3][d]);
+        if (std::fabs(T[1][d] * g - 1.0) > 1e-9 || std::fabs(T[4][d] * g - 1.0) > 1e-9 || off * g > 1e-9) {
+          *err = "split-precision path needs a conformal grid (G = sqrtG I); use CCS_FP64_PARITY";
+          return CCS_ERR_INVALID;
+        }
+      }
+  std::vector<double> sg(np), fs(np, 0.0);
+  for (std::size_t d = 0; d < np; ++d) {
+    sg[d] = T[0][d] / (r * r);
+    fs[d] = r * sg[d] * T[6][d];
+  }
+  std::vector<double> phis(T[5], T[5] + np);
+  std::vector<char> stage;
+  if (s->fp64) {
+    upload_cast<double>(st, s->sg, sg, stage);
+    upload_cast<double>(st, s->fs, fs, stage);
+    upload_cast<double>(st, s->phis, phis, stage);
+  } else {
+    upload_cast<float>(st, s->sg, sg, stage);
+    upload_cast<float>(st, s->fs, fs, stage);
+    upload_cast<float>(st, s->phis, phis, stage);
+  }
+  std::vector<double> sq(T[0], T[0] + np);
+  (void)hipMemcpyAsync(s->sqrtG_d, sq.data(), np * sizeof(double), hipMemcpyHostToDevice, st);
+  (void)hipMemcpyAsync(s->phis_d, phis.data(), np * sizeof(double), hipMemcpyHostToDevice, st);
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    *err = "split_set_mesh: upload failed";
+    return CCS_ERR_DEVICE;
+  }
+  s->sqrtG = std::move(sq);
+  s->IG11.assign(T[1], T[1] + np); s->IG12.assign(T[2], T[2] + np);
+  s->IG21.assign(T[3], T[3] + np); s->IG22.assign(T[4], T[4] + np);
+  s->M11.assign(T[7], T[7] + np); s->M12.assign(T[8], T[8] + np);
+  s->M21.assign(T[9], T[9] + np); s->M22.assign(T[10], T[10] + np);
+  s->mesh_set = true;
+  return CCS_OK;
+}
+
+int split_set_state(SplitState* s, hipStream_t st, const double* phiG, const double* u, const double* v,
+                    const double*, const double*, const double*, std::string* err) {
+  const std::size_t np = s->np;
+  const double r = s->cfg.radius;
+  s->pref64.assign(np, 0.0);
+  std::vector<double> U(np), V(np), zero(np, 0.0);
+  for (std::size_t d = 0; d < np; ++d) {
+    s->pref64[d] = phiG[d] / (r * r);
+    U[d] = u[d] / r;
+    V[d] = v[d] / r;
+  }
+  std::vector<char> stage;
+  if (s->fp64) {
+    upload_cast<double>(st, s->pref, s->pref64, stage);
+    upload_cast<double>(st, s->buf[0][0], zero, stage);
+    upload_cast<double>(st, s->buf[0][1], U, stage);
+    upload_cast<double>(st, s->buf[0][2], V, stage);
+  } else {
+    upload_cast<float>(st, s->pref, s->pref64, stage);
+    upload_cast<float>(st, s->buf[0][0], zero, stage);
+    upload_cast<float>(st, s->buf[0][1], U, stage);
+    upload_cast<float>(st, s->buf[0][2], V, stage);
+  }
+  (void)hipMemcpyAsync(s->pref64_d, s->pref64.data(), np * sizeof(double), hipMemcpyHostToDevice, st);
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    *err = "split_set_state: upload failed";
+    return CCS_ERR_DEVICE;
+  }
+  return CCS_OK;
+}
+
+int split_enqueue_step(SplitState* s, hipStream_t st, int scheme, bool, hipEvent_t* ev) {
+  if (s->fp64) enqueue_step_t<double>(s, st, scheme, ev);
+  else enqueue_step_t<float>(s, st, scheme, ev);
+  return CCS_OK;
+}
+
+int split_get_state(SplitState* s, hipStream_t st, bool, double* phiG, double* u, double* v, double* phi,
+                    double* uc, double* vc, double* zonal, double* merid, std::string* err) {
+  const int n = s->n, h = s->h;
+  const std::size_t np = s->np;
+  const double r = s->cfg.radius;
+  std::vector<double> dP, U, V;
+  if (s->fp64) {
+    download_to_double<double>(st, s->buf[0][0], np, dP);
+    download_to_double<double>(st, s->buf[0][1], np, U);
+    download_to_double<double>(st, s->buf[0][2], np, V);
+  } else {
+    download_to_double<float>(st, s->buf[0][0], np, dP);
+    download_to_double<float>(st, s->buf[0][1], np, U);
+    download_to_double<float>(st, s->buf[0][2], np, V);
+  }
+  if (hipGetLastError() != hipSuccess) {
+    *err = "split_get_state: download failed";
+    return CCS_ERR_DEVICE;
+  }
+  for (int p = 1; p <= 6; ++p)
+    for (int j = 1 - h; j <= n + h; ++j)
+      for (int i = 1 - h; i <= n + h; ++i) {
+        const std::size_t d = pidx(n, h, i, j, p);
+        const bool corner = is_corner(n, i, j);
+        const bool interior = i >= 1 && i <= n && j >= 1 && j <= n;
+        const double pg = corner ? 0.0 : r * r * (s->pref64[d] + dP[d]);
+        const double uu = corner ? 0.0 : r * U[d], vv = corner ? 0.0 : r * V[d];
+        if (phiG) phiG[d] = pg;
+        if (u) u[d] = uu;
+        if (v) v[d] = vv;
+        if (phi) phi[d] = corner ? 0.0 : pg / s->sqrtG[d];
+        if (uc) uc[d] = s->IG11[d] * uu + s->IG12[d] * vv;
+        if (vc) vc[d] = s->IG21[d] * uu + s->IG22[d] * vv;
+        if (zonal) zonal[d] = interior ? s->M11[d] * uu + s->M12[d] * vv : 0.0;
+        if (merid) merid[d] = interior ? s->M21[d] * uu + s->M22[d] * vv : 0.0;
+      }
+  return CCS_OK;
+}
+
+int split_tendency(SplitState* s, hipStream_t st, bool, double* tp, double* tu, double* tv, std::string* err) {
+  const double r = s->cfg.radius;
+  std::vector<double> t0, t1, t2;
+  if (s->fp64) {
+    SplitArgs<double> a = base_args<double>(s);
+    a.in = cprog_of<double>(s, 0); a.base = a.in;
+    for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<double*>(s->tend[k]);
+    launch_split<double>(st, kTendOnly, a, statics_of<double>(s), s->hs);
+    download_to_double<double>(st, s->tend[0], s->ni, t0);
+    download_to_double<double>(st, s->tend[1], s->ni, t1);
+    download_to_double<double>(st, s->tend[2], s->ni, t2);
+  } else {
+    SplitArgs<float> a = base_args<float>(s);
+    a.in = cprog_of<float>(s, 0); a.base = a.in;
+    for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<float*>(s->tend[k]);
+    launch_split<float>(st, kTendOnly, a, statics_of<float>(s), s->hs);
+    download_to_double<float>(st, s->tend[0], s->ni, t0);
+    download_to_double<float>(st, s->tend[1], s->ni, t1);
+    download_to_double<float>(st, s->tend[2], s->ni, t2);
+  }
+  if (hipGetLastError() != hipSuccess) {
+    *err = "split_tendency: kernel or copy failed";
+    return CCS_ERR_DEVICE;
+  }
+  // back to the reference variables: d(phiG)/dt = a dP/dtau, du/dt = dU/dtau
+  for (std::size_t k = 0; k < s->ni; ++k) {
+    if (tp) tp[k] = r * t0[k];
+    if (tu) tu[k] = t1[k];
+    if (tv) tv[k] = t2[k];
+  }
+  return CCS_OK;
+}
+
+int split_diagnostics(SplitState* s, hipStream_t st, bool, double* result, std::string* err) {
+  if (s->fp64) {
+    hipLaunchKernelGGL(split_diag<double>, dim3(kSplitDiagBlocks), dim3(kDT), 0, st, cprog_of<double>(s, 0),
+                       s->pref64_d, s->sqrtG_d, s->phis_d, s->n, s->h, s->cfg.radius, s->partials);
+  } else {
+    hipLaunchKernelGGL(split_diag<float>, dim3(kSplitDiagBlocks), dim3(kDT), 0, st, cprog_of<float>(s, 0),
+                       s->pref64_d, s->sqrtG_d, s->phis_d, s->n, s->h, s->cfg.radius, s->partials);
+  }
+  hipLaunchKernelGGL(split_diag_final, dim3(1), dim3(64), 0, st, s->partials, kSplitDiagBlocks, s->result);
+  if (hipMemcpyAsync(result, s->result, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    *err = "split_diagnostics: kernel or copy failed";
+    return CCS_ERR_DEVICE;
+  }
+  return CCS_OK;
+}
 
 }  // namespace ccs

@@ -1,0 +1,149 @@
+"""Split-precision path (FP32 state, FP64-derived metrics) and its FP64
+instantiation, against the FP64 oracle.
+
+The split kernels evaluate the reference equations in a non-dimensional
+form with the conformal-metric identities G = sqrtG I and a shared
+smoothness-indicator evaluation, so they agree with the reference to
+round-off of their own precision, not bit for bit:
+  - FP64 instantiation: relative differences at the 1e-12 level;
+  - FP32: the reference method itself amplifies any perturbation of size
+    ~1e-7 (its WENO-Z weight rule switches discontinuously,
+    spatial_operators_mod.F90:348-355, most of all near the panel corners),
+    so the FP32 state is held to the method's own floor: its distance from
+    the FP64 oracle must stay within a small factor of the distance between
+    two FP64 oracle runs whose initial states differ by FP32 rounding.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = 9.80616
+
+
+def _setup(ccs, n=32, halo=3, seed=3, mountain=True, hours=24.0):
+    mesh = ccs.Mesh(n, halo=halo, radius=6.371e6)
+    s0 = ccs.init_case(mesh, case_num=7, seed=seed, mountain=mountain)
+    dt, nsteps = ccs.cfl_time_step(mesh, s0, 0.5, hours * 3600.0)
+    return mesh, s0, dt, nsteps
+
+
+def _nl2(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+# FP32: the wind tendency of a near-balanced jet is the small residual of a
+# pressure-gradient and a Coriolis term each ~1e4 times larger, so FP32
+# rounding of those terms (6e-8) appears as ~1e-3 of the residual.
+@pytest.mark.parametrize("precision,tol", [("fp64_fast", 1e-11), ("fp32", 2e-3)])
+def test_split_tendency_close_to_oracle(ccs, orc, precision, tol):
+    mesh, s0, dt, _ = _setup(ccs)
+    print(f"split tendency {precision}", flush=True)
+    with ccs.DynamicalCore(mesh, dt, precision=precision) as core:
+        core.set_state(s0)
+        tg = core.spatial_operator()
+    to = orc.Oracle(mesh, s0, dt).tendency()
+    for name, a, b in zip(("phiG", "u", "v"), tg, to):
+        e = _nl2(a, b)
+        print(f"  L({name}) normalised L2 diff {e:.3e}")
+        assert e <= tol
+
+
+def test_split_fp64_one_day_vs_oracle(ccs, orc):
+    mesh, s0, dt, nsteps = _setup(ccs)
+    print(f"split fp64_fast: 1 day C32, {nsteps} steps", flush=True)
+    with ccs.DynamicalCore(mesh, dt, precision="fp64_fast") as core:
+        core.set_state(s0)
+        m0, _ = core.diagnostics()
+        core.step(nsteps)
+        s = core.get_state()
+        m1, _ = core.diagnostics()
+    o = orc.Oracle(mesh, s0, dt)
+    o.steps(nsteps)
+    area = mesh.interior(mesh.areaCell)
+    l1, l2, linf = ccs.error_norms(mesh.interior(s.phi) / G, mesh.interior(o.a["phi"]) / G, area)
+    print(f"  h: L1 {l1:.3e} L2 {l2:.3e} Linf {linf:.3e}; relative mass drift {(m1 - m0) / m0:.3e}")
+    assert l2 <= 1e-12
+    assert abs(m1 - m0) <= 1e-13 * abs(m0)
+
+
+@pytest.mark.parametrize("n,hours", [(32, 24.0), (96, 6.0)])
+def test_split_fp32_at_method_sensitivity_floor(ccs, orc, n, hours):
+    mesh, s0, dt, nsteps = _setup(ccs, n=n, hours=hours)
+    print(f"split fp32: C{n}, {hours} h, {nsteps} steps", flush=True)
+    with ccs.DynamicalCore(mesh, dt, precision="fp32") as core:
+        core.set_state(s0)
+        m0, _ = core.diagnostics()
+        core.step(nsteps)
+        s = core.get_state()
+        m1, _ = core.diagnostics()
+    o = orc.Oracle(mesh, s0, dt)
+    o.steps(nsteps)
+    # floor: oracle from an initial state perturbed at FP32 rounding level
+    rng = np.random.default_rng(2024)
+    sp = s0.copy()
+    for k in ("phiG", "u", "v"):
+        a = getattr(sp, k)
+        a *= 1.0 + 2.0 ** -24 * rng.standard_normal(a.shape)
+    op = orc.Oracle(mesh, sp, dt)
+    op.steps(nsteps)
+    area = mesh.interior(mesh.areaCell)
+    ref = mesh.interior(o.a["phi"]) / G
+    e32 = ccs.error_norms(mesh.interior(s.phi) / G, ref, area)
+    efl = ccs.error_norms(mesh.interior(op.a["phi"]) / G, ref, area)
+    print(f"  fp32 vs oracle      h: L1 {e32[0]:.3e} L2 {e32[1]:.3e} Linf {e32[2]:.3e}")
+    print(f"  perturbed vs oracle h: L1 {efl[0]:.3e} L2 {efl[1]:.3e} Linf {efl[2]:.3e}")
+    print(f"  relative mass drift {(m1 - m0) / m0:.3e}")
+    assert e32[1] <= 4.0 * efl[1] + 1e-9
+    assert abs(m1 - m0) <= 1e-9 * abs(m0)
+
+
+def orc_zonal(mesh, o):
+    A = mesh.A.reshape(-1, 2, 2)
+    IG = mesh.IG.reshape(-1, 2, 2)
+    u, v = o.a["u"], o.a["v"]
+    # covProjPlane2Sphere with matrix = matmul(A, IG); arrays are [cell, c, r]
+    M11 = A[:, 0, 0] * IG[:, 0, 0] + A[:, 1, 0] * IG[:, 0, 1]
+    M12 = A[:, 0, 0] * IG[:, 1, 0] + A[:, 1, 0] * IG[:, 1, 1]
+    return M11 * u + M12 * v
+
+
+def test_split_fp64_matches_parity_path_after_steps(ccs):
+    mesh, s0, dt, _ = _setup(ccs, n=48, halo=4, seed=9)
+    res = {}
+    for prec in ("fp64", "fp64_fast"):
+        print(f"{prec}: 20 steps C48 halo 4", flush=True)
+        with ccs.DynamicalCore(mesh, dt, precision=prec) as core:
+            core.set_state(s0)
+            core.step(20)
+            res[prec] = core.get_state()
+    mask = ~mesh.corner_mask().ravel()
+    for k in ("phiG", "u", "v"):
+        a, b = getattr(res["fp64_fast"], k)[mask], getattr(res["fp64"], k)[mask]
+        e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        print(f"  {k}: max rel diff {e:.3e}")
+        assert e <= 1e-11
+
+
+def test_split_rk4_runs_and_conserves_mass(ccs):
+    mesh, s0, dt, _ = _setup(ccs, n=32, seed=11)
+    with ccs.DynamicalCore(mesh, dt, precision="fp32", scheme="RK4") as core:
+        core.set_state(s0)
+        m0, _ = core.diagnostics()
+        core.step(50)
+        m1, _ = core.diagnostics()
+    assert abs(m1 - m0) <= 1e-9 * abs(m0)
+
+
+def test_split_rejects_nonconformal_grid(ccs):
+    mesh, s0, dt, _ = _setup(ccs, n=16)
+    bad = mesh.IG.copy()
+    bad[1::4] += 1e-3 * np.abs(bad[0::4])  # IG21 off-diagonal: not conformal
+    with ccs.DynamicalCore(mesh, dt, precision="fp32") as core:
+        mesh_ig = mesh.IG
+        try:
+            mesh.IG = bad
+            with pytest.raises(ccs.CcsError):
+                core.set_mesh(s0.phis)
+        finally:
+            mesh.IG = mesh_ig
