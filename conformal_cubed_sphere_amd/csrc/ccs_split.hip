@@ -82,48 +82,92 @@ struct SplitArgs {
 // sharing the smoothness indicators.  Weight rule of
 // spatial_operators_mod.F90:348-355; for the mirrored (left-face) window the
 // indicators reverse order, which swaps tau40 and tau41.
+//
+// With a_k, b_k the second and first differences of the Jiang-Shu
+// indicators, every candidate stencil is q2 + a_k/12 +- b_k/4:
+//   right face: s1 = q2 + a1/12 + b1/4, s2 = q2 + a2/12 - b2/4, s3 = q2 + a3/12 - b3/4
+//   left face : s1 = q2 + a3/12 + b3/4, s2 = q2 + a2/12 + b2/4, s3 = q2 + a1/12 - b1/4
+// and the branch weights (1/3,2/3,0), (0,2/3,1/3) are the unnormalised
+// weights (1,2,0), (0,2,1) fed through the same normalisation.
 template <typename T>
-__device__ __forceinline__ void weno_pair(T q0, T q1, T q2, T q3, T q4, T eps, T& plus, T& minus) {
+struct WenoWin {
+  T c1, c2, c3;     // q2 + a_k/12
+  T h1, h2, h3;     // b_k/2
+  T t40, t41, mb;   // indicator differences and min
+  T tr1, tr2, tr3;  // tau5 / (eps + beta_k)
+};
+
+template <typename T>
+__device__ __forceinline__ WenoWin<T> weno_window(T q0, T q1, T q2, T q3, T q4, T eps) {
 #pragma clang fp contract(fast)
-  const T a1 = q0 - T(2) * q1 + q2;
-  const T a2 = q1 - T(2) * q2 + q3;
-  const T a3 = q2 - T(2) * q3 + q4;
-  const T b1 = q0 - T(4) * q1 + T(3) * q2;
-  const T b2 = q1 - q3;
-  const T b3 = T(3) * q2 - T(4) * q3 + q4;
-  const T c13 = T(13) / T(12);
-  const T be1 = c13 * a1 * a1 + T(0.25) * b1 * b1;
-  const T be2 = c13 * a2 * a2 + T(0.25) * b2 * b2;
-  const T be3 = c13 * a3 * a3 + T(0.25) * b3 * b3;
-  const T t40 = fabs(be1 - be2);
-  const T t41 = fabs(be2 - be3);
+  WenoWin<T> w;
+  const T a1 = T(-2) * q1 + (q0 + q2);
+  const T a2 = T(-2) * q2 + (q1 + q3);
+  const T a3 = T(-2) * q3 + (q2 + q4);
+  w.h1 = T(0.5) * a1 + (q2 - q1);
+  w.h2 = T(0.5) * (q1 - q3);
+  w.h3 = T(0.5) * a3 + (q2 - q3);
+  const T k13 = T(13) / T(12);
+  const T be1 = (k13 * a1) * a1 + w.h1 * w.h1;
+  const T be2 = (k13 * a2) * a2 + w.h2 * w.h2;
+  const T be3 = (k13 * a3) * a3 + w.h3 * w.h3;
+  w.t40 = fabs(be1 - be2);
+  w.t41 = fabs(be2 - be3);
   const T t5 = fabs(be3 - be1);
-  const T mb = fmin(be1, fmin(be2, be3));
-  const T big = T(1e30);
-  const T tr1 = fmin(t5 * rcp(eps + be1), big);
-  const T tr2 = fmin(t5 * rcp(eps + be2), big);
-  const T tr3 = fmin(t5 * rcp(eps + be3), big);
-  const T s6 = T(1) / T(6);
-  {
-    const T s1 = (T(2) * q0 - T(7) * q1 + T(11) * q2) * s6;
-    const T s2 = (-q1 + T(5) * q2 + T(2) * q3) * s6;
-    const T s3 = (T(2) * q2 + T(5) * q3 - q4) * s6;
-    const T al1 = T(0.1) + T(0.1) * tr1, al2 = T(0.6) + T(0.6) * tr2, al3 = T(0.3) + T(0.3) * tr3;
-    const T gen = (al1 * s1 + al2 * s2 + al3 * s3) * rcp(al1 + al2 + al3);
-    const T br1 = (s1 + T(2) * s2) * (T(1) / T(3));
-    const T br2 = (T(2) * s2 + s3) * (T(1) / T(3));
-    plus = (t40 <= mb && t41 > mb) ? br1 : ((t40 > mb && t41 < mb) ? br2 : gen);
-  }
-  {
-    const T s1 = (T(2) * q4 - T(7) * q3 + T(11) * q2) * s6;
-    const T s2 = (-q3 + T(5) * q2 + T(2) * q1) * s6;
-    const T s3 = (T(2) * q2 + T(5) * q1 - q0) * s6;
-    const T al1 = T(0.1) + T(0.1) * tr3, al2 = T(0.6) + T(0.6) * tr2, al3 = T(0.3) + T(0.3) * tr1;
-    const T gen = (al1 * s1 + al2 * s2 + al3 * s3) * rcp(al1 + al2 + al3);
-    const T br1 = (s1 + T(2) * s2) * (T(1) / T(3));
-    const T br2 = (T(2) * s2 + s3) * (T(1) / T(3));
-    minus = (t41 <= mb && t40 > mb) ? br1 : ((t41 > mb && t40 < mb) ? br2 : gen);
-  }
+  w.mb = fmin(be1, fmin(be2, be3));
+  w.tr1 = t5 * rcp(eps + be1);
+  w.tr2 = t5 * rcp(eps + be2);
+  w.tr3 = t5 * rcp(eps + be3);
+  const T k12 = T(1) / T(12);
+  w.c1 = a1 * k12 + q2;
+  w.c2 = a2 * k12 + q2;
+  w.c3 = a3 * k12 + q2;
+  return w;
+}
+
+template <typename T>
+__device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T w1, T w2, T w3, bool br1, bool br2) {
+#pragma clang fp contract(fast)
+  w1 = br1 ? T(1) : (br2 ? T(0) : w1);
+  w2 = (br1 || br2) ? T(2) : w2;
+  w3 = br1 ? T(0) : (br2 ? T(1) : w3);
+  return (w1 * s1 + w2 * s2 + w3 * s3) * rcp(w1 + w2 + w3);
+}
+
+// value at the right face of the window's centre cell
+template <typename T>
+__device__ __forceinline__ T weno_right(const WenoWin<T>& w) {
+#pragma clang fp contract(fast)
+  const T s1 = w.c1 + T(0.5) * w.h1, s2 = w.c2 - T(0.5) * w.h2, s3 = w.c3 - T(0.5) * w.h3;
+  const bool b1 = (w.t40 <= w.mb) && (w.t41 > w.mb);
+  const bool b2 = (w.t40 > w.mb) && (w.t41 < w.mb);
+  return weno_combine(s1, s2, s3, T(0.1) + T(0.1) * w.tr1, T(0.6) + T(0.6) * w.tr2, T(0.3) + T(0.3) * w.tr3, b1,
+                      b2);
+}
+
+// value at the left face of the window's centre cell (mirrored window)
+template <typename T>
+__device__ __forceinline__ T weno_left(const WenoWin<T>& w) {
+#pragma clang fp contract(fast)
+  const T s1 = w.c3 + T(0.5) * w.h3, s2 = w.c2 + T(0.5) * w.h2, s3 = w.c1 - T(0.5) * w.h1;
+  const bool b1 = (w.t41 <= w.mb) && (w.t40 > w.mb);
+  const bool b2 = (w.t41 > w.mb) && (w.t40 < w.mb);
+  return weno_combine(s1, s2, s3, T(0.1) + T(0.1) * w.tr3, T(0.6) + T(0.6) * w.tr2, T(0.3) + T(0.3) * w.tr1, b1,
+                      b2);
+}
+
+// either face, chosen per lane (left = true: the left face)
+template <typename T>
+__device__ __forceinline__ T weno_side(const WenoWin<T>& w, bool left) {
+#pragma clang fp contract(fast)
+  const T s1 = left ? w.c3 + T(0.5) * w.h3 : w.c1 + T(0.5) * w.h1;
+  const T s2 = w.c2 + (left ? T(0.5) : T(-0.5)) * w.h2;
+  const T s3 = left ? w.c1 - T(0.5) * w.h1 : w.c3 - T(0.5) * w.h3;
+  const T ta = left ? w.t41 : w.t40, tb = left ? w.t40 : w.t41;
+  const T ra = left ? w.tr3 : w.tr1, rb = left ? w.tr1 : w.tr3;
+  const bool b1 = (ta <= w.mb) && (tb > w.mb);
+  const bool b2 = (ta > w.mb) && (tb < w.mb);
+  return weno_combine(s1, s2, s3, T(0.1) + T(0.1) * ra, T(0.6) + T(0.6) * w.tr2, T(0.3) + T(0.3) * rb, b1, b2);
 }
 
 template <typename T>
@@ -135,10 +179,28 @@ __device__ __forceinline__ T d6f(T m3, T m2, T m1, T p1, T p2, T p3, T rdx) {
 template <typename T, int KIND>
 __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
 #pragma clang fp contract(fast)
-  __shared__ T sFx[NE], sFy[NE], sE[NE], sPt[NE], sUC[NE], sVC[NE], sPh[NE], sRg[NE], sU[NE], sV[NE];
+  // cell fields on the extended tile: 0 Fx, 1 Fy, 2 E, 3 phitG, 4 Uc, 5 Vc,
+  // 6 phit, 7 1/sqrtg, 8 U, 9 V
+  __shared__ T sc[10 * NE];
+  T* const sFx = sc;
+  T* const sFy = sc + NE;
+  T* const sE = sc + 2 * NE;
+  T* const sPt = sc + 3 * NE;
+  T* const sUC = sc + 4 * NE;
+  T* const sVC = sc + 5 * NE;
+  T* const sPh = sc + 6 * NE;
+  T* const sRg = sc + 7 * NE;
+  T* const sU = sc + 8 * NE;
+  T* const sV = sc + 9 * NE;
+  // face values: x-faces [TY][TX+1], y-faces [TY+1][TX]; per variable
+  // (0 F, 1 E, 2 phitG) the right-face (+) and left-face (-) reconstructions
   constexpr int NFXF = TY * (TX + 1);
   constexpr int NFYF = (TY + 1) * TX;
-  __shared__ T xp[3][NFXF], xm[3][NFXF], yp[3][NFYF], ym[3][NFYF];
+  __shared__ T sfc[6 * NFXF + 6 * NFYF];
+  T(*const xp)[NFXF] = reinterpret_cast<T(*)[NFXF]>(sfc);
+  T(*const xm)[NFXF] = reinterpret_cast<T(*)[NFXF]>(sfc + 3 * NFXF);
+  T(*const yp)[NFYF] = reinterpret_cast<T(*)[NFYF]>(sfc + 6 * NFXF);
+  T(*const ym)[NFYF] = reinterpret_cast<T(*)[NFYF]>(sfc + 6 * NFXF + 3 * NFYF);
 
   const int n = a.n, h = a.h, P = n + 2 * h;
   const int p = blockIdx.z;
@@ -182,75 +244,95 @@ __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T
   }
   __syncthreads();
 
-  // ---- phase 2a: one task per (window centre, variable)
-  constexpr int NWX = (TX + 2) * TY;
-  constexpr int NWY = TX * (TY + 2);
-  for (int e = tid; e < 3 * (NWX + NWY); e += NT) {
-    if (e < 3 * NWX) {
-      const int var = e / NWX, w = e % NWX;
-      const int ty = w / (TX + 2), c = w % (TX + 2) - 1;  // centre cell, tile-local
-      const int ig = i0 + c, j = j0 + ty;
-      if (j >= n || ig < -1 || ig > n) continue;
-      const T* s = (var == 0) ? sFx : ((var == 1) ? sE : sPt);
+  // ---- phase 2a: WENO windows.  Thread (tx, ty) owns the windows centred on
+  // its own cell along x and along y for each of the three reconstructed
+  // variables; the windows centred one cell outside the tile (needed by the
+  // tile's boundary faces) form one extra pass.  Windows of cells outside
+  // the panel are evaluated on whatever the LDS holds and never used.
+  const int tx = tid % TX, ty = tid / TX;
+  {
+    const int l = (ty + 3) * EX + tx + 3;
+    const int fx = ty * (TX + 1) + tx;   // left x-face of the cell
+    const int fy = ty * TX + tx;         // bottom y-face of the cell
+#pragma unroll
+    for (int var = 0; var < 3; ++var) {
+      const T* sv = (var == 0) ? sFx : ((var == 1) ? sE : sPt);
       const T eps = (var == 0) ? a.epsF : ((var == 1) ? a.epsE : a.epsP);
-      const int l = (ty + 3) * EX + c + 3;
-      T pl, mi;
-      weno_pair(s[l - 2], s[l - 1], s[l], s[l + 1], s[l + 2], eps, pl, mi);
-      if (c + 1 <= TX) xp[var][ty * (TX + 1) + c + 1] = pl;  // right face of c
-      if (c >= 0) xm[var][ty * (TX + 1) + c] = mi;             // left face of c
-    } else {
-      const int e2 = e - 3 * NWX;
-      const int var = e2 / NWY, w = e2 % NWY;
-      const int tx = w % TX, c = w / TX - 1;
-      const int i = i0 + tx, jg = j0 + c;
-      if (i >= n || jg < -1 || jg > n) continue;
-      const T* s = (var == 0) ? sFy : ((var == 1) ? sE : sPt);
+      const WenoWin<T> wx = weno_window(sv[l - 2], sv[l - 1], sv[l], sv[l + 1], sv[l + 2], eps);
+      xp[var][fx + 1] = weno_right(wx);
+      xm[var][fx] = weno_left(wx);
+      const T* sw = (var == 0) ? sFy : ((var == 1) ? sE : sPt);
+      const WenoWin<T> wy = weno_window(sw[l - 2 * EX], sw[l - EX], sw[l], sw[l + EX], sw[l + 2 * EX], eps);
+      yp[var][fy + TX] = weno_right(wy);
+      ym[var][fy] = weno_left(wy);
+    }
+  }
+  {
+    // extra windows, one uniform code path: x: centres -1 (right face -> x-face
+    // 0) and TX (left face -> x-face TX) of each row; y: centres -1 and TY of
+    // each column.  Each lane gets an LDS base, a stride and an output slot.
+    constexpr int NXE = 2 * TY, NYE = 2 * TX, NPER = NXE + NYE, NEXTRA = 3 * NPER;
+    for (int e = tid; e < NEXTRA; e += NT) {
+      const int var = e / NPER;
+      const int k = e % NPER;
+      int l0, stride, out;
+      bool left;
+      if (k < NXE) {
+        const int row = k >> 1;
+        left = (k & 1) != 0;
+        l0 = (row + 3) * EX + (left ? TX : -1) + 3 + (var == 0 ? 0 : (var + 1) * NE);
+        stride = 1;
+        out = left ? (3 + var) * NFXF + row * (TX + 1) + TX : var * NFXF + row * (TX + 1);
+      } else {
+        const int k2 = k - NXE;
+        const int col = k2 % TX;
+        left = k2 >= TX;
+        l0 = ((left ? TY : -1) + 3) * EX + col + 3 + (var == 0 ? NE : (var + 1) * NE);
+        stride = EX;
+        out = 6 * NFXF + (left ? (3 + var) * NFYF + TY * TX + col : var * NFYF + col);
+      }
       const T eps = (var == 0) ? a.epsF : ((var == 1) ? a.epsE : a.epsP);
-      const int l = (c + 3) * EX + tx + 3;
-      T pl, mi;
-      weno_pair(s[l - 2 * EX], s[l - EX], s[l], s[l + EX], s[l + 2 * EX], eps, pl, mi);
-      if (c + 1 <= TY) yp[var][(c + 1) * TX + tx] = pl;
-      if (c >= 0) ym[var][c * TX + tx] = mi;
+      const WenoWin<T> w = weno_window(sc[l0 - 2 * stride], sc[l0 - stride], sc[l0], sc[l0 + stride],
+                                       sc[l0 + 2 * stride], eps);
+      sfc[out] = weno_side(w, left);
     }
   }
   __syncthreads();
 
-  // ---- phase 2b: Rusanov fluxes (results overwrite xp[0]/xp[1], yp[0]/yp[1])
-  for (int e = tid; e < NFXF + NFYF; e += NT) {
-    if (e < NFXF) {
-      const int fi = e % (TX + 1), ty = e / (TX + 1);
-      const int ig = i0 + fi - 1, j = j0 + ty;
-      if (j >= n || ig < -1 || ig > n - 1) continue;
-      const int l = (ty + 3) * EX + fi + 2;  // cell ig; face between l and l+1
-      const T ub = T(0.5) * (sUC[l] + sUC[l + 1]);
-      const T pb = T(0.5) * (sPh[l] + sPh[l + 1]);
-      const T gb = T(0.5) * (sRg[l] + sRg[l + 1]);
-      const T lam = fabs(ub) + sqrt(gb * pb);
-      const int f = e;
-      const T F = T(0.5) * (xp[0][f] + xm[0][f] - lam * (xm[2][f] - xp[2][f]));
-      const T E = T(0.5) * (xp[1][f] + xm[1][f]);
-      xp[0][f] = F;
-      xp[1][f] = E;
-    } else {
-      const int f = e - NFXF;
-      const int tx = f % TX, fj = f / TX;
-      const int i = i0 + tx, jg = j0 + fj - 1;
-      if (i >= n || jg < -1 || jg > n - 1) continue;
-      const int l = (fj + 2) * EX + tx + 3;
-      const T vb = T(0.5) * (sVC[l] + sVC[l + EX]);
-      const T pb = T(0.5) * (sPh[l] + sPh[l + EX]);
-      const T gb = T(0.5) * (sRg[l] + sRg[l + EX]);
-      const T lam = fabs(vb) + sqrt(gb * pb);
-      const T F = T(0.5) * (yp[0][f] + ym[0][f] - lam * (ym[2][f] - yp[2][f]));
-      const T E = T(0.5) * (yp[1][f] + ym[1][f]);
-      yp[0][f] = F;
-      yp[1][f] = E;
-    }
-  }
+  // ---- phase 2b: Rusanov fluxes; thread (tx, ty) does the right x-face and
+  // the top y-face of its cell, the tile's left and bottom faces are extra.
+  // Results overwrite xp[0]/xp[1] (x) and yp[0]/yp[1] (y).
+  auto xface = [&](int fi, int row) {
+    const int l = (row + 3) * EX + fi + 2;  // cell left of the face
+    const T ub = T(0.5) * (sUC[l] + sUC[l + 1]);
+    const T pb = T(0.5) * (sPh[l] + sPh[l + 1]);
+    const T gb = T(0.5) * (sRg[l] + sRg[l + 1]);
+    const T lam = fabs(ub) + sqrt(gb * pb);
+    const int f = row * (TX + 1) + fi;
+    const T F = T(0.5) * (xp[0][f] + xm[0][f] - lam * (xm[2][f] - xp[2][f]));
+    const T E = T(0.5) * (xp[1][f] + xm[1][f]);
+    xp[0][f] = F;
+    xp[1][f] = E;
+  };
+  auto yface = [&](int col, int fj) {
+    const int l = (fj + 2) * EX + col + 3;  // cell below the face
+    const T vb = T(0.5) * (sVC[l] + sVC[l + EX]);
+    const T pb = T(0.5) * (sPh[l] + sPh[l + EX]);
+    const T gb = T(0.5) * (sRg[l] + sRg[l + EX]);
+    const T lam = fabs(vb) + sqrt(gb * pb);
+    const int f = fj * TX + col;
+    const T F = T(0.5) * (yp[0][f] + ym[0][f] - lam * (ym[2][f] - yp[2][f]));
+    const T E = T(0.5) * (yp[1][f] + ym[1][f]);
+    yp[0][f] = F;
+    yp[1][f] = E;
+  };
+  xface(tx + 1, ty);
+  yface(tx, ty + 1);
+  if (tid < TY) xface(0, tid);
+  else if (tid >= 64 && tid < 64 + TX) yface(tid - 64, 0);
   __syncthreads();
 
   // ---- phase 3: cells
-  const int tx = tid % TX, ty = tid / TX;
   const int i = i0 + tx, j = j0 + ty;
   if (i >= n || j >= n) return;
   const T rdx = a.rdx;
