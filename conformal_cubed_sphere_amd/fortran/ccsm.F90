@@ -1,0 +1,302 @@
+! ccsm.F90 -- driver of the MI355X shallow-water dynamical core.
+!
+! Drop-in for the reference driver (dynamic_core/src/ccsm.F90:2-73): same
+! namelist.input groups (&time_settings, &case_select, &domain,
+! &dynamic_opt of parameters_mod.F90:87-100, same defaults :118-129), same
+! initialisation order (initParameters, read_mesh, initStat/initTend,
+! initTestCase), the same step loop with history cadence
+! (ccsm.F90:50-67) and the same "output index/total, MCR, ECR" line.  The
+! time step itself runs on the GPU through libccs.so (ccs_gpu_mod).
+!
+! Differences, all in the optional group &gpu_opt that follows &dynamic_opt:
+!   nx         cells per panel edge.  The reference reads the grid from a
+!              netCDF mesh file (io_mod.F90:37-117); there is no netCDF here,
+!              so the conformal mesh is generated in memory (ccs_mesh_geometry).
+!   precision  'fp64' (bit-faithful to the reference), 'fp32', 'fp64_fast'.
+!   cfl        > 0: dt = largest step <= cfl * min(dx / max Rusanov speed)
+!              that divides the run, from the initial state; when the
+!              history interval divides the run, the step count is rounded
+!              up to a multiple of the number of outputs.
+!   seed, n_bumps, mountain, u0, h_mean: case 7, the seeded balanced jet.
+!   radius, gravity, omega, device, use_graph, final_state_file.
+module ccs_params_mod
+  use iso_c_binding
+  implicit none
+  ! &time_settings
+  real(8) :: dt
+  integer :: run_days, run_hours, run_minutes, run_seconds, history_interval
+  character(len=200) :: integral_scheme
+  ! &case_select
+  integer :: case_num
+  ! &domain
+  character(len=200) :: mesh_file
+  integer :: xhalo, yhalo
+  ! &dynamic_opt
+  character(len=200) :: reconstruct_scheme = 'WENO'
+  ! &gpu_opt
+  integer :: nx = 0
+  character(len=16) :: precision = 'fp64'
+  real(8) :: cfl = 0.d0
+  integer :: seed = 1, n_bumps = 8, mountain = 0
+  real(8) :: u0 = 20.d0, h_mean = 10000.d0
+  real(8) :: radius = 6371229.d0, gravity = 9.80616d0, omega = 7.292d-5
+  integer :: device = 0, use_graph = 1
+  character(len=200) :: final_state_file = ''
+
+  integer :: total_run_time, nsteps
+  integer :: steps_per_output = 0
+
+  namelist /time_settings/ dt, run_days, run_hours, run_minutes, run_seconds, history_interval, integral_scheme
+  namelist /case_select/ case_num
+  namelist /domain/ mesh_file, xhalo, yhalo
+  namelist /dynamic_opt/ reconstruct_scheme
+  namelist /gpu_opt/ nx, precision, cfl, seed, n_bumps, mountain, u0, h_mean, radius, gravity, omega, device, &
+                     use_graph, final_state_file
+
+contains
+
+  subroutine initParameters
+    integer :: ios
+    dt = 300.d0
+    xhalo = 1
+    yhalo = 1
+    run_days = 1
+    run_hours = 0
+    run_minutes = 0
+    run_seconds = 0
+    history_interval = 360
+    integral_scheme = 'RK4'
+    case_num = 2
+    mesh_file = 'ccs_mesh'
+    open(1, file='namelist.input', status='old')
+    read(1, nml=time_settings)
+    read(1, nml=case_select)
+    read(1, nml=domain)
+    read(1, nml=dynamic_opt)
+    read(1, nml=gpu_opt, iostat=ios)
+    close(1)
+    total_run_time = run_days * 86400 + run_hours * 3600 + run_minutes * 60 + run_seconds
+    if (trim(adjustl(integral_scheme)) /= 'RK3_TVD' .and. trim(adjustl(integral_scheme)) /= 'RK4') then
+      error stop 'Unknown integral scheme, please select from RK3_TVD or RK4 ...'
+    end if
+    if (trim(adjustl(reconstruct_scheme)) /= 'WENO') error stop 'Only reconstruct_scheme = WENO is implemented'
+    if (xhalo /= yhalo) error stop 'xhalo and yhalo must be equal'
+    if (xhalo < 3) error stop 'xhalo >= 3 is required (WENO-5 faces and the 6th-order vorticity)'
+    if (nx < 4 .or. mod(nx, 2) /= 0) error stop 'set nx (even, >= 4) in &gpu_opt: the netCDF mesh file cannot be read'
+    if (history_interval <= 0) error stop 'history_interval must be positive'
+    nsteps = int(total_run_time / dt)
+  end subroutine initParameters
+
+end module ccs_params_mod
+
+program ccsm
+  use iso_c_binding
+  use ccs_gpu_mod
+  use ccs_params_mod
+  implicit none
+
+  integer :: n, h, pw, np, ni, it, nxt, k, output_idx, total_output_num
+  integer(8) :: t_start, t_end, t_loop0, t_loop1, rate
+  real(8), allocatable, target :: lon_deg(:), lat_deg(:), xyz(:), jab(:)
+  real(8), allocatable, target :: lon(:), lat(:), sinlon(:), coslon(:), sinlat(:), coslat(:)
+  real(8), allocatable, target :: sqrtG(:), f(:), area(:), G(:), IG(:), A(:), IA(:)
+  real(8), allocatable :: phiG(:), u(:), v(:), phi(:), uc(:), vc(:), zonal(:), merid(:), phis(:)
+  type(ccs_metrics) :: m
+  type(ccs_case_params) :: cp
+  type(ccs_config) :: cfg
+  type(c_ptr) :: hnd
+  real(8) :: mass0, energy0, mass, energy, dummy, loop_s, area_sum
+  character(len=*), parameter :: fmt_out = '(1x,a,i8,a,i8,2es25.16)'
+
+  call system_clock(t_start, rate)
+  print *, 'Start the MCV Shallow Water model (MI355X, ', c_to_f_string(ccs_version()), ')'
+  call initParameters
+
+  ! ---- read_mesh (io_mod.F90:15-307): generated conformal mesh + static tables
+  n = nx
+  h = xhalo
+  pw = n + 2 * h
+  np = pw * pw * 6
+  ni = n * n * 6
+  allocate(lon_deg(ni), lat_deg(ni), xyz(3 * ni), jab(6 * ni))
+  call ccs_check(ccs_mesh_geometry(n, lon_deg, lat_deg, xyz, jab), 'ccs_mesh_geometry')
+  allocate(lon(np), lat(np), sinlon(np), coslon(np), sinlat(np), coslat(np), sqrtG(np), f(np), area(np))
+  allocate(G(4 * np), IG(4 * np), A(4 * np), IA(4 * np))
+  m%n = n
+  m%halo = h
+  m%radius = radius
+  m%omega = omega
+  m%lon = c_loc(lon); m%lat = c_loc(lat); m%sinlon = c_loc(sinlon); m%coslon = c_loc(coslon)
+  m%sinlat = c_loc(sinlat); m%coslat = c_loc(coslat); m%sqrtG = c_loc(sqrtG); m%f = c_loc(f)
+  m%areaCell = c_loc(area); m%G = c_loc(G); m%IG = c_loc(IG); m%A = c_loc(A); m%IA = c_loc(IA)
+  call ccs_check(ccs_mesh_metrics(lon_deg, lat_deg, jab, m), 'ccs_mesh_metrics')
+  area_sum = interior_sum(area)
+  print *, 'Spherical area, numerical, analytical :', area_sum, 4.d0 * acos(-1.d0) * radius**2
+
+  ! ---- initTestCase (test_case_mod.F90:13-58)
+  allocate(phiG(np), u(np), v(np), phi(np), uc(np), vc(np), zonal(np), merid(np), phis(np))
+  call ccs_case_defaults(cp)
+  cp%case_num = case_num
+  cp%seed = int(seed, c_long_long)
+  cp%n_bumps = n_bumps
+  cp%mountain = mountain
+  cp%gravity = gravity
+  cp%u0 = u0
+  cp%h_mean = h_mean
+  call ccs_check(ccs_init_case(m, cp, phiG, u, v, phi, uc, vc, zonal, merid, phis), 'ccs_init_case')
+  print *, ''
+  print *, 'test case ', case_num, ' is selected, C', n, ' halo ', h
+  if (cfl > 0.d0) then
+    call cfl_dt(cfl, dble(total_run_time), dt, nsteps)
+    ! keep the history cadence on whole steps: round the step count up to a
+    ! multiple of the number of outputs and test outputs by step index
+    if (mod(total_run_time, history_interval) == 0) then
+      k = total_run_time / history_interval
+      nsteps = ((nsteps + k - 1) / k) * k
+      dt = dble(total_run_time) / nsteps
+      steps_per_output = nsteps / k
+    end if
+    print *, 'dt from CFL ', cfl, ': ', dt, ' s, ', nsteps, ' steps'
+  end if
+  print *, ''
+  print *, 'Temporal integration scheme is ' // trim(adjustl(integral_scheme)), ', precision ', trim(precision)
+  print *, ''
+
+  ! ---- GPU state (initStat/initTend)
+  cfg%n = n
+  cfg%halo = h
+  select case (trim(precision))
+  case ('fp64')
+    cfg%precision = CCS_FP64_PARITY
+  case ('fp32')
+    cfg%precision = CCS_FP32_SPLIT
+  case ('fp64_fast')
+    cfg%precision = CCS_FP64_FAST
+  case default
+    error stop 'precision must be fp64, fp32 or fp64_fast'
+  end select
+  cfg%scheme = merge(CCS_RK3_TVD, CCS_RK4, trim(adjustl(integral_scheme)) == 'RK3_TVD')
+  cfg%eflux = CCS_EFLUX_NONE
+  cfg%device = device
+  cfg%use_graph = use_graph
+  cfg%pad_ = 0
+  cfg%dt = dt
+  cfg%dx = m%dx
+  cfg%radius = radius
+  cfg%gravity = gravity
+  cfg%omega = omega
+  call ccs_check(ccs_create(cfg, hnd), 'ccs_create')
+  call ccs_check(ccs_set_mesh(hnd, sqrtG, G, IG, A, IA, f, phis), 'ccs_set_mesh', hnd)
+  call ccs_check(ccs_set_state(hnd, phiG, u, v, phi, uc, vc), 'ccs_set_state', hnd)
+
+  output_idx = 1
+  total_output_num = int(nsteps * dt / history_interval)
+  if (steps_per_output > 0) total_output_num = nsteps / steps_per_output
+  call ccs_check(ccs_diagnostics(hnd, mass0, energy0), 'ccs_diagnostics', hnd)
+  write(*, fmt_out) 'output index/total, MCR, ECR :', output_idx - 1, '/', total_output_num, 0.d0, 0.d0
+
+  ! ---- time integration (ccsm.F90:50-67): the steps between two outputs go
+  ! to the GPU in one call; at an output the energy is taken on the state
+  ! before the last step, as the reference does with stat(old) (:62).
+  call system_clock(t_loop0)
+  it = 0
+  do while (it < nsteps)
+    nxt = nsteps
+    do k = it + 1, nsteps
+      if (is_output(k)) then
+        nxt = k
+        exit
+      end if
+    end do
+    if (is_output(nxt)) then
+      if (nxt - it > 1) call ccs_check(ccs_step(hnd, nxt - it - 1), 'ccs_step', hnd)
+      call ccs_check(ccs_diagnostics(hnd, dummy, energy), 'ccs_diagnostics', hnd)
+      call ccs_check(ccs_step(hnd, 1), 'ccs_step', hnd)
+      call ccs_check(ccs_diagnostics(hnd, mass, dummy), 'ccs_diagnostics', hnd)
+      output_idx = output_idx + 1
+      write(*, fmt_out) 'output index/total, MCR, ECR :', output_idx - 1, '/', total_output_num, &
+               (mass - mass0) / mass0, (energy - energy0) / energy0
+    else
+      call ccs_check(ccs_step(hnd, nxt - it), 'ccs_step', hnd)
+    end if
+    it = nxt
+  end do
+  call ccs_check(ccs_synchronize(hnd), 'ccs_synchronize', hnd)
+  call system_clock(t_loop1)
+  loop_s = dble(t_loop1 - t_loop0) / dble(rate)
+  print *, 'Step loop: ', nsteps, ' steps in ', loop_s, ' s, cell-updates/s ', dble(ni) * nsteps / max(loop_s, 1d-9)
+
+  if (len_trim(final_state_file) > 0) then
+    call ccs_check(ccs_get_state(hnd, phiG, u, v, phi, uc, vc, zonal, merid), 'ccs_get_state', hnd)
+    open(2, file=trim(final_state_file), access='stream', form='unformatted', status='replace')
+    write(2) n, h, nsteps, dt
+    write(2) phiG, u, v, phi
+    close(2)
+  end if
+  call ccs_destroy(hnd)
+
+  call system_clock(t_end)
+  print *, 'It took ', dble(t_end - t_start) / dble(rate), ' seconds to run this program'
+
+contains
+
+  logical function is_output(k)
+    integer, intent(in) :: k
+    if (steps_per_output > 0) then
+      is_output = mod(k, steps_per_output) == 0
+    else
+      is_output = (mod(k * dt, dble(history_interval)) == 0.d0) .and. (k * dt >= history_interval)
+    end if
+  end function is_output
+
+  real(8) function interior_sum(fld)
+    real(8), intent(in) :: fld(:)
+    integer :: i, j, p
+    interior_sum = 0.d0
+    do p = 1, 6
+      do j = 1, n
+        do i = 1, n
+          interior_sum = interior_sum + fld((i + h) + pw * ((j + h - 1) + pw * (p - 1)))
+        end do
+      end do
+    end do
+  end function interior_sum
+
+  ! Largest dt <= c * min(dx / max lambda) dividing the run, lambda the
+  ! Rusanov face speed of spatial_operators_mod.F90:109-116,134-141.
+  subroutine cfl_dt(c, total, dt_out, nst)
+    real(8), intent(in) :: c, total
+    real(8), intent(out) :: dt_out
+    integer, intent(out) :: nst
+    integer :: i, j, p, d0, d1
+    real(8) :: lx, ly, ub, pb, gb
+    lx = 0.d0
+    ly = 0.d0
+    do p = 1, 6
+      do j = 1, n
+        do i = 0, n
+          d0 = (i + h) + pw * ((j + h - 1) + pw * (p - 1))
+          d1 = d0 + 1
+          ub = 0.5d0 * (uc(d1) + uc(d0))
+          pb = 0.5d0 * (phi(d1) + phis(d1) + phi(d0) + phis(d0))
+          gb = 0.5d0 * (IG(4 * (d1 - 1) + 1) + IG(4 * (d0 - 1) + 1))
+          lx = max(lx, abs(ub) + sqrt(gb * pb))
+        end do
+      end do
+      do j = 0, n
+        do i = 1, n
+          d0 = (i + h) + pw * ((j + h - 1) + pw * (p - 1))
+          d1 = d0 + pw
+          ub = 0.5d0 * (vc(d1) + vc(d0))
+          pb = 0.5d0 * (phi(d1) + phis(d1) + phi(d0) + phis(d0))
+          gb = 0.5d0 * (IG(4 * (d1 - 1) + 4) + IG(4 * (d0 - 1) + 4))
+          ly = max(ly, abs(ub) + sqrt(gb * pb))
+        end do
+      end do
+    end do
+    dt_out = c * min(m%dx / lx, m%dx / ly)
+    nst = ceiling(total / dt_out - 1d-12)
+    dt_out = total / nst
+  end subroutine cfl_dt
+
+end program ccsm

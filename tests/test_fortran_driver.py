@@ -1,0 +1,127 @@
+"""The Fortran host (conformal_cubed_sphere_amd/fortran/ccsm.exe): the
+reference driver's namelist and step loop (ccsm.F90:2-73) calling the GPU
+library through iso_c_binding."""
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "conformal_cubed_sphere_amd", "fortran", "ccsm.exe")
+
+NAMELIST = """&time_settings
+ dt               = {dt}
+ run_days         = 0
+ run_hours        = {hours}
+ run_minutes      = 0
+ run_seconds      = 0
+ history_interval = {hist}
+ integral_scheme  = '{scheme}'
+/
+
+&case_select
+ case_num = {case}
+/
+
+&domain
+ mesh_file = 'unused.nc'
+ xhalo     = {halo}
+ yhalo     = {halo}
+/
+
+&dynamic_opt
+ reconstruct_scheme = 'WENO'
+/
+
+&gpu_opt
+{gpu}
+/
+"""
+
+
+def _run(tmp_path, timeout=120, **kw):
+    args = dict(dt=600.0, hours=6, hist=3600, scheme="RK3_TVD", case=7, halo=3, gpu=" nx = 32")
+    args.update(kw)
+    (tmp_path / "namelist.input").write_text(NAMELIST.format(**args))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=timeout)
+    return r.returncode, r.stdout + r.stderr
+
+
+def _need_exe():
+    if not os.path.exists(EXE):
+        pytest.fail("ccsm.exe is not built: run __graft_entry__.build()")
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (dict(scheme="Euler"), "Unknown integral scheme"),
+    (dict(gpu=" precision = 'fp64'"), "set nx"),
+    (dict(gpu=" nx = 33"), "set nx"),
+    (dict(halo=2), "xhalo >= 3"),
+])
+def test_namelist_validation(tmp_path, bad, msg):
+    _need_exe()
+    rc, out = _run(tmp_path, **bad)
+    assert rc != 0 and msg in out, out
+
+
+def test_mesh_area_and_cfl_step_before_the_device(tmp_path, ccs):
+    """Everything before the first device call runs on the host: the
+    generated mesh, the seeded case, and the CFL step (same as the Python
+    host's).  Without a GPU the driver must stop loudly at ccs_create."""
+    _need_exe()
+    rc, out = _run(tmp_path, gpu=" nx = 32\n cfl = 0.5\n seed = 3")
+    m = ccs.Mesh(32, halo=3)
+    s = ccs.init_case(m, case_num=7, seed=3)
+    dt, n = _cfl_with_outputs(ccs, m, s, 6)
+    area = re.search(r"Spherical area, numerical, analytical :\s*(\S+)\s+(\S+)", out)
+    assert area and abs(float(area.group(1)) / float(np.sum(m.interior(m.areaCell))) - 1) < 1e-13, out
+    got = re.search(r"dt from CFL\s+\S+\s*:\s*(\S+)\s+s,\s+(\d+)\s+steps", out)
+    assert got and float(got.group(1)) == pytest.approx(dt, rel=1e-15) and int(got.group(2)) == n, out
+    if "ccs error in ccs_create" in out:
+        assert rc == 2
+    else:
+        assert rc == 0, out
+
+
+def _cfl_with_outputs(ccs, m, s, hours):
+    # CFL step of the Python host, step count rounded up to whole hours
+    _, n = ccs.cfl_time_step(m, s, 0.5, hours * 3600.0)
+    n = -(-n // hours) * hours
+    return hours * 3600.0 / n, n
+
+
+def _read_final(path, npad):
+    b = path.read_bytes()
+    n, h, nsteps, dt = struct.unpack("<iiid", b[:20])
+    f = np.frombuffer(b[20:], dtype="<f8")
+    assert f.size == 4 * npad
+    return n, h, nsteps, dt, {k: f[i * npad:(i + 1) * npad] for i, k in enumerate(("phiG", "u", "v", "phi"))}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,code", [("RK3_TVD", 0), ("RK4", 1)])
+def test_driver_run_matches_oracle_bitwise(tmp_path, ccs, orc, scheme, code):
+    """C32, seeded jet with mountain, 6 hours at CFL 0.5 with hourly output:
+    the FP64 path through the Fortran host equals the oracle bit for bit, and
+    the printed mass change stays at round-off."""
+    _need_exe()
+    rc, out = _run(tmp_path, scheme=scheme,
+                   gpu=" nx = 32\n cfl = 0.5\n seed = 5\n mountain = 1\n final_state_file = 'final.bin'")
+    print(out)
+    assert rc == 0, out
+    m = ccs.Mesh(32, halo=3)
+    n, h, nsteps, dt, fin = _read_final(tmp_path / "final.bin", m.npad)
+    assert (n, h) == (32, 3)
+    s0 = ccs.init_case(m, case_num=7, seed=5, mountain=True)
+    dt_py, n_py = _cfl_with_outputs(ccs, m, s0, 6)
+    assert dt == dt_py and nsteps == n_py
+    o = orc.Oracle(m, s0, dt)
+    o.steps(nsteps, scheme=code)
+    mask = ~m.corner_mask().ravel()
+    for k in ("phiG", "u", "v", "phi"):
+        assert np.array_equal(fin[k][mask], o.a[k][mask]), k
+    mcr = [float(x.split()[-2]) for x in out.splitlines() if "MCR, ECR" in x]
+    assert len(mcr) == 7 and max(abs(x) for x in mcr) < 1e-13
