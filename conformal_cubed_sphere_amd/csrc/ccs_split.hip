@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "ccs_internal.h"
@@ -43,12 +44,10 @@
 namespace ccs {
 namespace {
 
+// Tile: TX x TY interior cells per workgroup, one thread per cell.  TX is
+// fixed at 32 (half a wavefront per row); TY is a template parameter.
 constexpr int TX = 32;
-constexpr int TY = 8;
-constexpr int NT = TX * TY;
 constexpr int EX = TX + 6;
-constexpr int EY = TY + 6;
-constexpr int NE = EX * EY;
 
 __device__ __forceinline__ long gidx(int i, int j, int p, int h, int P) {
   return static_cast<long>(i + h) + static_cast<long>(P) * (static_cast<long>(j + h) + static_cast<long>(P) * p);
@@ -56,6 +55,21 @@ __device__ __forceinline__ long gidx(int i, int j, int p, int h, int P) {
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+
+// Loads and stores at a 32-bit byte offset from a uniform base pointer: the
+// compiler then addresses every array of a cell with one shared VGPR offset
+// (global_load ... vOff, sBase) instead of a 64-bit address per array.  The
+// host guarantees that every array of the split state is below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* p, unsigned off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* p, unsigned off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
+}
 
 template <typename T>
 struct SplitStatics {
@@ -72,6 +86,7 @@ struct SplitArgs {
   T* tend[3];
   const T* tacc[9];
   int n, h;
+  int nbx, nby;        // tiles per panel along x and y
   T rdx;               // 1 / dx
   T dts, incs;         // dt / a, RK4 increment / a
   T epsF, epsE, epsP;  // WENO-Z epsilons of Fx/Fy, E, phitG/a^2
@@ -94,7 +109,7 @@ struct WenoWin {
   T c1, c2, c3;     // q2 + a_k/12
   T h1, h2, h3;     // b_k/2
   T t40, t41, mb;   // indicator differences and min
-  T tr1, tr2, tr3;  // tau5 / (eps + beta_k)
+  T tr1, tr2, tr3;  // 1 + tau5 / (eps + beta_k)
 };
 
 template <typename T>
@@ -115,9 +130,9 @@ __device__ __forceinline__ WenoWin<T> weno_window(T q0, T q1, T q2, T q3, T q4, 
   w.t41 = fabs(be2 - be3);
   const T t5 = fabs(be3 - be1);
   w.mb = fmin(be1, fmin(be2, be3));
-  w.tr1 = t5 * rcp(eps + be1);
-  w.tr2 = t5 * rcp(eps + be2);
-  w.tr3 = t5 * rcp(eps + be3);
+  w.tr1 = t5 * rcp(eps + be1) + T(1);
+  w.tr2 = t5 * rcp(eps + be2) + T(1);
+  w.tr3 = t5 * rcp(eps + be3) + T(1);
   const T k12 = T(1) / T(12);
   w.c1 = a1 * k12 + q2;
   w.c2 = a2 * k12 + q2;
@@ -125,12 +140,24 @@ __device__ __forceinline__ WenoWin<T> weno_window(T q0, T q1, T q2, T q3, T q4, 
   return w;
 }
 
+// WENO-Z combination of the three candidates with the linear weights
+// (d1, d2, d3) = (0.1, 0.6, 0.3) and u_k = 1 + tau5 / (eps + beta_k):
+// w_k = d_k u_k.  The branch cases of the weight rule (br1: (1/3, 2/3, 0),
+// br2: (0, 2/3, 1/3)) replace the weights by (1, 2, 0) or (0, 2, 1) before the
+// same normalisation.  They occur only at local extrema of the indicators, so
+// the replacement sits behind a branch that whole wavefronts usually skip.
 template <typename T>
-__device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T w1, T w2, T w3, bool br1, bool br2) {
+__device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T u1, T u2, T u3, bool br1, bool br2) {
 #pragma clang fp contract(fast)
-  w1 = br1 ? T(1) : (br2 ? T(0) : w1);
-  w2 = (br1 || br2) ? T(2) : w2;
-  w3 = br1 ? T(0) : (br2 ? T(1) : w3);
+  T w1 = T(0.1) * u1, w2 = T(0.6) * u2, w3 = T(0.3) * u3;
+  if (br1 || br2) {
+    // an empty volatile asm keeps this a branch (skipped by whole
+    // wavefronts) instead of being if-converted into per-lane selects
+    asm volatile("");
+    w1 = br1 ? T(1) : T(0);
+    w2 = T(2);
+    w3 = br1 ? T(0) : T(1);
+  }
   return (w1 * s1 + w2 * s2 + w3 * s3) * rcp(w1 + w2 + w3);
 }
 
@@ -141,8 +168,7 @@ __device__ __forceinline__ T weno_right(const WenoWin<T>& w) {
   const T s1 = w.c1 + T(0.5) * w.h1, s2 = w.c2 - T(0.5) * w.h2, s3 = w.c3 - T(0.5) * w.h3;
   const bool b1 = (w.t40 <= w.mb) && (w.t41 > w.mb);
   const bool b2 = (w.t40 > w.mb) && (w.t41 < w.mb);
-  return weno_combine(s1, s2, s3, T(0.1) + T(0.1) * w.tr1, T(0.6) + T(0.6) * w.tr2, T(0.3) + T(0.3) * w.tr3, b1,
-                      b2);
+  return weno_combine(s1, s2, s3, w.tr1, w.tr2, w.tr3, b1, b2);
 }
 
 // value at the left face of the window's centre cell (mirrored window)
@@ -152,8 +178,7 @@ __device__ __forceinline__ T weno_left(const WenoWin<T>& w) {
   const T s1 = w.c3 + T(0.5) * w.h3, s2 = w.c2 + T(0.5) * w.h2, s3 = w.c1 - T(0.5) * w.h1;
   const bool b1 = (w.t41 <= w.mb) && (w.t40 > w.mb);
   const bool b2 = (w.t41 > w.mb) && (w.t40 < w.mb);
-  return weno_combine(s1, s2, s3, T(0.1) + T(0.1) * w.tr3, T(0.6) + T(0.6) * w.tr2, T(0.3) + T(0.3) * w.tr1, b1,
-                      b2);
+  return weno_combine(s1, s2, s3, w.tr3, w.tr2, w.tr1, b1, b2);
 }
 
 // either face, chosen per lane (left = true: the left face)
@@ -167,7 +192,7 @@ __device__ __forceinline__ T weno_side(const WenoWin<T>& w, bool left) {
   const T ra = left ? w.tr3 : w.tr1, rb = left ? w.tr1 : w.tr3;
   const bool b1 = (ta <= w.mb) && (tb > w.mb);
   const bool b2 = (ta > w.mb) && (tb < w.mb);
-  return weno_combine(s1, s2, s3, T(0.1) + T(0.1) * ra, T(0.6) + T(0.6) * w.tr2, T(0.3) + T(0.3) * rb, b1, b2);
+  return weno_combine(s1, s2, s3, ra, w.tr2, rb, b1, b2);
 }
 
 template <typename T>
@@ -176,9 +201,13 @@ __device__ __forceinline__ T d6f(T m3, T m2, T m1, T p1, T p2, T p3, T rdx) {
   return ((p3 - m3) - T(9) * (p2 - m2) + T(45) * (p1 - m1)) * (rdx * (T(1) / T(60)));
 }
 
-template <typename T, int KIND>
-__global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
+template <typename T, int KIND, int TY>
+__global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
 #pragma clang fp contract(fast)
+  constexpr int NT = TX * TY;
+  constexpr int EY = TY + 6;
+  constexpr int NE = EX * EY;
+  constexpr unsigned W = sizeof(T);
   // cell fields on the extended tile: 0 Fx, 1 Fy, 2 E, 3 phitG, 4 Uc, 5 Vc,
   // 6 phit, 7 1/sqrtg, 8 U, 9 V
   __shared__ T sc[10 * NE];
@@ -202,10 +231,25 @@ __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T
   T(*const yp)[NFYF] = reinterpret_cast<T(*)[NFYF]>(sfc + 6 * NFXF);
   T(*const ym)[NFYF] = reinterpret_cast<T(*)[NFYF]>(sfc + 6 * NFXF + 3 * NFYF);
 
-  const int n = a.n, h = a.h, P = n + 2 * h;
-  const int p = blockIdx.z;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+  // XCDs, each with its own L2, so workgroup b runs on XCD b % 8.  Each XCD
+  // is given a contiguous run of tiles (x fastest, then y, then panel) so the
+  // border rows and columns that neighbouring tiles share are fetched into
+  // one L2 instead of up to three.
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned xq = nb >> 3, xr = nb & 7u, xcd = b & 7u;
+  const unsigned t = xcd * xq + min(xcd, xr) + (b >> 3);
+  const unsigned tq = t / static_cast<unsigned>(a.nbx);
+  const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
+  const int i0 = static_cast<int>(t - tq * a.nbx) * TX;
+  const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
+
+  const int n = a.n, h = a.h;
+  const unsigned P = static_cast<unsigned>(n + 2 * h);
   const int tid = threadIdx.x;
+  // padded index of the extended tile's (0, 0) cell (i0 - 3, j0 - 3); h >= 3
+  const unsigned gt = static_cast<unsigned>(p) * P * P + static_cast<unsigned>(j0 - 3 + h) * P +
+                      static_cast<unsigned>(i0 - 3 + h);
 
   // ---- phase 1
   constexpr int NX = EX * TY;
@@ -223,13 +267,13 @@ __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T
     const int i = i0 + li - 3, j = j0 + lj - 3;
     if (i < -h || i >= n + h || j < -h || j >= n + h) continue;
     if ((i < 0 || i >= n) && (j < 0 || j >= n)) continue;
-    const long g = gidx(i, j, p, h, P);
-    const T Pv = st.pref[g] + a.in.phiG[g];
-    const T U = a.in.u[g], V = a.in.v[g];
-    const T sg = st.sg[g];
+    const unsigned go = (gt + __umul24(static_cast<unsigned>(lj), P) + static_cast<unsigned>(li)) * W;
+    const T Pv = ldo(st.pref, go) + ldo(a.in.phiG, go);
+    const T U = ldo(a.in.u, go), V = ldo(a.in.v, go);
+    const T sg = ldo(st.sg, go);
     const T rg = rcp(sg);
     const T Uc = U * rg, Vc = V * rg;
-    const T phit = Pv * rg + st.phis[g];
+    const T phit = Pv * rg + ldo(st.phis, go);
     const int l = lj * EX + li;
     sE[l] = phit + T(0.5) * (U * Uc + V * Vc);
     sFx[l] = Pv * Uc;
@@ -307,7 +351,7 @@ __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T
     const T ub = T(0.5) * (sUC[l] + sUC[l + 1]);
     const T pb = T(0.5) * (sPh[l] + sPh[l + 1]);
     const T gb = T(0.5) * (sRg[l] + sRg[l + 1]);
-    const T lam = fabs(ub) + sqrt(gb * pb);
+    const T lam = fabs(ub) + fsqrt(gb * pb);
     const int f = row * (TX + 1) + fi;
     const T F = T(0.5) * (xp[0][f] + xm[0][f] - lam * (xm[2][f] - xp[2][f]));
     const T E = T(0.5) * (xp[1][f] + xm[1][f]);
@@ -319,7 +363,7 @@ __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T
     const T vb = T(0.5) * (sVC[l] + sVC[l + EX]);
     const T pb = T(0.5) * (sPh[l] + sPh[l + EX]);
     const T gb = T(0.5) * (sRg[l] + sRg[l + EX]);
-    const T lam = fabs(vb) + sqrt(gb * pb);
+    const T lam = fabs(vb) + fsqrt(gb * pb);
     const int f = fj * TX + col;
     const T F = T(0.5) * (yp[0][f] + ym[0][f] - lam * (ym[2][f] - yp[2][f]));
     const T E = T(0.5) * (yp[1][f] + ym[1][f]);
@@ -328,6 +372,7 @@ __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T
   };
   xface(tx + 1, ty);
   yface(tx, ty + 1);
+  static_assert(TY <= 64 && TX <= 64, "extra faces: x in wave 0, y in wave 1");
   if (tid < TY) xface(0, tid);
   else if (tid >= 64 && tid < 64 + TX) yface(tid - 64, 0);
   __syncthreads();
@@ -344,63 +389,72 @@ __global__ void __launch_bounds__(NT) stage_split(SplitArgs<T> a, SplitStatics<T
   const T dEy = (yp[1][fy + TX] - yp[1][fy]) * rdx;
   const T dvdx = d6f(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], rdx);
   const T dudy = d6f(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX], sU[lc + 3 * EX], rdx);
-  const long g = gidx(i, j, p, h, P);
-  const T zeta = dvdx - dudy + st.fs[g];
+  const unsigned go = (gt + __umul24(static_cast<unsigned>(ty + 3), P) + static_cast<unsigned>(tx + 3)) * W;
+  const T zeta = dvdx - dudy + ldo(st.fs, go);
   const T tp = -(dFx + dFy);
   const T tu = -dEx + zeta * sVC[lc];
   const T tv = -dEy - zeta * sUC[lc];
-  const long ii = static_cast<long>(i) + static_cast<long>(n) * (static_cast<long>(j) + static_cast<long>(n) * p);
+  const unsigned un_ = static_cast<unsigned>(n);
+  const unsigned io = (static_cast<unsigned>(p) * un_ * un_ + static_cast<unsigned>(j0) * un_ +
+                       static_cast<unsigned>(i0) + __umul24(static_cast<unsigned>(ty), un_) +
+                       static_cast<unsigned>(tx)) * W;
   if (KIND == kTendOnly) {
-    a.tend[0][ii] = tp;
-    a.tend[1][ii] = tu;
-    a.tend[2][ii] = tv;
+    sto(a.tend[0], io, tp);
+    sto(a.tend[1], io, tu);
+    sto(a.tend[2], io, tv);
     return;
   }
   T pn, un, vn;
   if (KIND == kRK3s1) {
-    pn = a.base.phiG[g] + a.dts * tp;
-    un = a.base.u[g] + a.dts * tu;
-    vn = a.base.v[g] + a.dts * tv;
+    pn = ldo(a.base.phiG, go) + a.dts * tp;
+    un = ldo(a.base.u, go) + a.dts * tu;
+    vn = ldo(a.base.v, go) + a.dts * tv;
   } else if (KIND == kRK3s2) {
     const T w = T(0.25) * a.dts;
-    pn = T(0.75) * a.base.phiG[g] + T(0.25) * a.in.phiG[g] + w * tp;
-    un = T(0.75) * a.base.u[g] + T(0.25) * a.in.u[g] + w * tu;
-    vn = T(0.75) * a.base.v[g] + T(0.25) * a.in.v[g] + w * tv;
+    pn = T(0.75) * ldo(a.base.phiG, go) + T(0.25) * ldo(a.in.phiG, go) + w * tp;
+    un = T(0.75) * ldo(a.base.u, go) + T(0.25) * ldo(a.in.u, go) + w * tu;
+    vn = T(0.75) * ldo(a.base.v, go) + T(0.25) * ldo(a.in.v, go) + w * tv;
   } else if (KIND == kRK3s3) {
     const T c3 = T(1) / T(3), c23 = T(2) / T(3);
     const T w = c23 * a.dts;
-    pn = c3 * a.base.phiG[g] + c23 * a.in.phiG[g] + w * tp;
-    un = c3 * a.base.u[g] + c23 * a.in.u[g] + w * tu;
-    vn = c3 * a.base.v[g] + c23 * a.in.v[g] + w * tv;
+    pn = c3 * ldo(a.base.phiG, go) + c23 * ldo(a.in.phiG, go) + w * tp;
+    un = c3 * ldo(a.base.u, go) + c23 * ldo(a.in.u, go) + w * tu;
+    vn = c3 * ldo(a.base.v, go) + c23 * ldo(a.in.v, go) + w * tv;
   } else if (KIND == kRK4a) {
-    a.tend[0][ii] = tp;
-    a.tend[1][ii] = tu;
-    a.tend[2][ii] = tv;
-    pn = a.base.phiG[g] + a.incs * tp;
-    un = a.base.u[g] + a.incs * tu;
-    vn = a.base.v[g] + a.incs * tv;
+    sto(a.tend[0], io, tp);
+    sto(a.tend[1], io, tu);
+    sto(a.tend[2], io, tv);
+    pn = ldo(a.base.phiG, go) + a.incs * tp;
+    un = ldo(a.base.u, go) + a.incs * tu;
+    vn = ldo(a.base.v, go) + a.incs * tv;
   } else {
     const T c6 = T(1) / T(6);
-    pn = a.base.phiG[g] + a.dts * ((a.tacc[0][ii] + T(2) * (a.tacc[3][ii] + a.tacc[6][ii]) + tp) * c6);
-    un = a.base.u[g] + a.dts * ((a.tacc[1][ii] + T(2) * (a.tacc[4][ii] + a.tacc[7][ii]) + tu) * c6);
-    vn = a.base.v[g] + a.dts * ((a.tacc[2][ii] + T(2) * (a.tacc[5][ii] + a.tacc[8][ii]) + tv) * c6);
+    pn = ldo(a.base.phiG, go) +
+         a.dts * ((ldo(a.tacc[0], io) + T(2) * (ldo(a.tacc[3], io) + ldo(a.tacc[6], io)) + tp) * c6);
+    un = ldo(a.base.u, go) +
+         a.dts * ((ldo(a.tacc[1], io) + T(2) * (ldo(a.tacc[4], io) + ldo(a.tacc[7], io)) + tu) * c6);
+    vn = ldo(a.base.v, go) +
+         a.dts * ((ldo(a.tacc[2], io) + T(2) * (ldo(a.tacc[5], io) + ldo(a.tacc[8], io)) + tv) * c6);
   }
-  a.out.phiG[g] = pn;
-  a.out.u[g] = un;
-  a.out.v[g] = vn;
+  sto(a.out.phiG, go, pn);
+  sto(a.out.u, go, un);
+  sto(a.out.v, go, vn);
   const int i1 = i + 1, j1 = j + 1;
   const bool strip[4] = {i1 <= h, i1 >= n - h + 1, j1 <= h, j1 >= n - h + 1};
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (!strip[s]) continue;
     const int* r = hs.r[p][s];
-    const int id = r[1] + r[2] * i1 + r[3] * j1;
-    const int jd = r[4] + r[5] * i1 + r[6] * j1;
-    const long gd = gidx(id - 1, jd - 1, r[0] - 1, h, P);
+    // halo destination (1-based cell id, jd of panel r[0]); the coefficients
+    // r[2], r[3], r[5], r[6] are 0 or +-1
+    const int id = r[1] + __mul24(r[2], i1) + __mul24(r[3], j1);
+    const int jd = r[4] + __mul24(r[5], i1) + __mul24(r[6], j1);
+    const unsigned gd = (static_cast<unsigned>(r[0] - 1) * P * P + __umul24(static_cast<unsigned>(jd - 1 + h), P) +
+                         static_cast<unsigned>(id - 1 + h)) * W;
     const T c[2] = {un, vn};
-    a.out.phiG[gd] = pn;
-    a.out.u[gd] = static_cast<T>(r[8]) * c[r[7]];
-    a.out.v[gd] = static_cast<T>(r[10]) * c[r[9]];
+    sto(a.out.phiG, gd, pn);
+    sto(a.out.u, gd, static_cast<T>(r[8]) * c[r[7]]);
+    sto(a.out.v, gd, static_cast<T>(r[10]) * c[r[9]]);
   }
 }
 
@@ -464,18 +518,29 @@ __global__ void split_diag_final(const double* partials, int nb, double* result)
   result[1] = E.s + (E.c + ce);
 }
 
-template <typename T>
-void launch_split(hipStream_t s, int kind, const SplitArgs<T>& a, const SplitStatics<T>& st, const HaloScatter& hs) {
-  const dim3 grid((a.n + TX - 1) / TX, (a.n + TY - 1) / TY, 6);
-  const dim3 block(NT);
+template <typename T, int TY>
+void launch_split_ty(hipStream_t s, int kind, const SplitArgs<T>& a, const SplitStatics<T>& st,
+                     const HaloScatter& hs) {
+  const dim3 grid(static_cast<unsigned>(a.nbx * a.nby * 6));
+  const dim3 block(TX * TY);
   switch (kind) {
-    case kTendOnly: hipLaunchKernelGGL((stage_split<T, kTendOnly>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s1: hipLaunchKernelGGL((stage_split<T, kRK3s1>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s2: hipLaunchKernelGGL((stage_split<T, kRK3s2>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s3: hipLaunchKernelGGL((stage_split<T, kRK3s3>), grid, block, 0, s, a, st, hs); break;
-    case kRK4a: hipLaunchKernelGGL((stage_split<T, kRK4a>), grid, block, 0, s, a, st, hs); break;
-    default: hipLaunchKernelGGL((stage_split<T, kRK4f>), grid, block, 0, s, a, st, hs); break;
+    case kTendOnly: hipLaunchKernelGGL((stage_split<T, kTendOnly, TY>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_split<T, kRK3s1, TY>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_split<T, kRK3s2, TY>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_split<T, kRK3s3, TY>), grid, block, 0, s, a, st, hs); break;
+    case kRK4a: hipLaunchKernelGGL((stage_split<T, kRK4a, TY>), grid, block, 0, s, a, st, hs); break;
+    default: hipLaunchKernelGGL((stage_split<T, kRK4f, TY>), grid, block, 0, s, a, st, hs); break;
   }
+}
+
+// Tile heights instantiated; the launch fills nbx, nby for the chosen one.
+template <typename T>
+void launch_split(hipStream_t s, int kind, SplitArgs<T> a, const SplitStatics<T>& st, const HaloScatter& hs,
+                  int tile_y) {
+  a.nbx = (a.n + TX - 1) / TX;
+  a.nby = (a.n + tile_y - 1) / tile_y;
+  if (tile_y == 16) launch_split_ty<T, 16>(s, kind, a, st, hs);
+  else launch_split_ty<T, 8>(s, kind, a, st, hs);
 }
 
 }  // namespace
@@ -486,6 +551,7 @@ struct SplitState {
   HaloScatter hs{};
   bool fp64 = false;
   int n = 0, h = 0;
+  int tile_y = 8;  // stage-kernel tile height (8 or 16)
   std::size_t np = 0, ni = 0, esz = 4;
   void* dmem = nullptr;
   // typed device arrays (float or double)
@@ -540,7 +606,7 @@ void enqueue_step_t(SplitState* s, hipStream_t st, int scheme, hipEvent_t* ev) {
   int stage = 0;
   auto launch = [&](int kind) {
     if (ev) (void)hipEventRecord(ev[2 * stage], st);
-    launch_split<T>(st, kind, a, S, s->hs);
+    launch_split<T>(st, kind, a, S, s->hs, s->tile_y);
     if (ev) (void)hipEventRecord(ev[2 * stage + 1], st);
     ++stage;
   };
@@ -599,6 +665,21 @@ int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out,
   s->h = cfg.halo;
   s->np = padded_cells(cfg.n, cfg.halo);
   s->ni = static_cast<std::size_t>(cfg.n) * cfg.n * 6;
+  // the stage kernel addresses each array with a 32-bit byte offset
+  if (s->np * s->esz >= (std::size_t{1} << 32)) {
+    *err = "split_create: a padded field exceeds 4 GiB; n is too large for the split path";
+    delete s;
+    return CCS_ERR_INVALID;
+  }
+  if (const char* ty = std::getenv("CCS_SPLIT_TILE_Y")) {
+    const int v = std::atoi(ty);
+    if (v != 8 && v != 16) {
+      *err = "CCS_SPLIT_TILE_Y must be 8 or 16";
+      delete s;
+      return CCS_ERR_INVALID;
+    }
+    s->tile_y = v;
+  }
   // typed: 4 statics + 9 state (padded) + 9 tendencies (interior)
   const std::size_t typed = (4 + 9) * s->np + 9 * s->ni;
   const std::size_t dbl = 3 * s->np + 4 * kSplitDiagBlocks + 2;
@@ -765,7 +846,7 @@ int split_tendency(SplitState* s, hipStream_t st, bool, double* tp, double* tu, 
     SplitArgs<double> a = base_args<double>(s);
     a.in = cprog_of<double>(s, 0); a.base = a.in;
     for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<double*>(s->tend[k]);
-    launch_split<double>(st, kTendOnly, a, statics_of<double>(s), s->hs);
+    launch_split<double>(st, kTendOnly, a, statics_of<double>(s), s->hs, s->tile_y);
     download_to_double<double>(st, s->tend[0], s->ni, t0);
     download_to_double<double>(st, s->tend[1], s->ni, t1);
     download_to_double<double>(st, s->tend[2], s->ni, t2);
@@ -773,7 +854,7 @@ int split_tendency(SplitState* s, hipStream_t st, bool, double* tp, double* tu, 
     SplitArgs<float> a = base_args<float>(s);
     a.in = cprog_of<float>(s, 0); a.base = a.in;
     for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<float*>(s->tend[k]);
-    launch_split<float>(st, kTendOnly, a, statics_of<float>(s), s->hs);
+    launch_split<float>(st, kTendOnly, a, statics_of<float>(s), s->hs, s->tile_y);
     download_to_double<float>(st, s->tend[0], s->ni, t0);
     download_to_double<float>(st, s->tend[1], s->ni, t1);
     download_to_double<float>(st, s->tend[2], s->ni, t2);
