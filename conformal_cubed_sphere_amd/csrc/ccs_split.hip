@@ -24,12 +24,16 @@
 //
 // Kernel structure (one launch per RK stage, as the FP64 path):
 //   phase 1  pointwise terms on the tile and a 3-cell cross border -> LDS
-//   phase 2a one task per (window centre, variable): the smoothness
-//            indicators of the 5-point window are computed once and used by
-//            both reconstructions the window feeds (the + value at its right
-//            face and the - value at its left face; the reference recomputes
-//            them per face, spatial_operators_mod.F90:80-99)
-//   phase 2b Rusanov flux per face
+//   phase 2  WENO windows: the smoothness indicators of each 5-point window
+//            are computed once and feed both reconstructions the window
+//            gives (the value at its centre cell's right face and at its left
+//            face; the reference recomputes them per face,
+//            spatial_operators_mod.F90:80-99).  The Rusanov flux of a face,
+//            F = (F_L + F_R)/2 - lambda (P_R - P_L)/2, is split into the half
+//            each side contributes, (F_L + lambda P_L)/2 and
+//            (F_R - lambda P_R)/2, so every window writes its own share and
+//            no separate flux pass (or barrier) is needed; lambda is computed
+//            from the cell values beside the face by each contributor.
 //   phase 3  divergence, vorticity, RK update, halo scatter with the signed
 //            permutation of the panel-basis change (ghost_mod.f90:100-153).
 #include <hip/hip_runtime.h>
@@ -208,28 +212,31 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
   constexpr int EY = TY + 6;
   constexpr int NE = EX * EY;
   constexpr unsigned W = sizeof(T);
-  // cell fields on the extended tile: 0 Fx, 1 Fy, 2 E, 3 phitG, 4 Uc, 5 Vc,
-  // 6 phit, 7 1/sqrtg, 8 U, 9 V
-  __shared__ T sc[10 * NE];
+  // cell fields on the extended tile: 0 Fx, 1 Fy, 2 E, 3 phitG, 4 phit,
+  // 5 1/sqrtg, 6 U, 7 V
+  __shared__ T sc[8 * NE];
   T* const sFx = sc;
   T* const sFy = sc + NE;
   T* const sE = sc + 2 * NE;
   T* const sPt = sc + 3 * NE;
-  T* const sUC = sc + 4 * NE;
-  T* const sVC = sc + 5 * NE;
-  T* const sPh = sc + 6 * NE;
-  T* const sRg = sc + 7 * NE;
-  T* const sU = sc + 8 * NE;
-  T* const sV = sc + 9 * NE;
-  // face values: x-faces [TY][TX+1], y-faces [TY+1][TX]; per variable
-  // (0 F, 1 E, 2 phitG) the right-face (+) and left-face (-) reconstructions
+  T* const sPh = sc + 4 * NE;
+  T* const sRg = sc + 5 * NE;
+  T* const sU = sc + 6 * NE;
+  T* const sV = sc + 7 * NE;
+  // face shares: x-faces [TY][TX+1], y-faces [TY+1][TX]; per face the share
+  // of the flux of phitG*u (F) and of E from its left/lower side (p) and its
+  // right/upper side (m)
   constexpr int NFXF = TY * (TX + 1);
   constexpr int NFYF = (TY + 1) * TX;
-  __shared__ T sfc[6 * NFXF + 6 * NFYF];
-  T(*const xp)[NFXF] = reinterpret_cast<T(*)[NFXF]>(sfc);
-  T(*const xm)[NFXF] = reinterpret_cast<T(*)[NFXF]>(sfc + 3 * NFXF);
-  T(*const yp)[NFYF] = reinterpret_cast<T(*)[NFYF]>(sfc + 6 * NFXF);
-  T(*const ym)[NFYF] = reinterpret_cast<T(*)[NFYF]>(sfc + 6 * NFXF + 3 * NFYF);
+  __shared__ T sfc[4 * NFXF + 4 * NFYF];
+  T* const xpF = sfc;
+  T* const xmF = sfc + NFXF;
+  T* const xpE = sfc + 2 * NFXF;
+  T* const xmE = sfc + 3 * NFXF;
+  T* const ypF = sfc + 4 * NFXF;
+  T* const ymF = ypF + NFYF;
+  T* const ypE = ypF + 2 * NFYF;
+  T* const ymE = ypF + 3 * NFYF;
 
   // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
   // XCDs, each with its own L2, so workgroup b runs on XCD b % 8.  Each XCD
@@ -281,100 +288,110 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
     sPt[l] = phit * sg;
     sPh[l] = phit;
     sRg[l] = rg;
-    sUC[l] = Uc;
-    sVC[l] = Vc;
     sU[l] = U;
     sV[l] = V;
   }
   __syncthreads();
 
-  // ---- phase 2a: WENO windows.  Thread (tx, ty) owns the windows centred on
-  // its own cell along x and along y for each of the three reconstructed
-  // variables; the windows centred one cell outside the tile (needed by the
-  // tile's boundary faces) form one extra pass.  Windows of cells outside
-  // the panel are evaluated on whatever the LDS holds and never used.
+  // Rusanov speed at the x-face between extended-tile cells l and l + 1, and
+  // at the y-face between l and l + EX (spatial_operators_mod.F90:109-116,
+  // 134-141, on a conformal grid)
+  auto lam_x = [&](int l) {
+    const T ub = T(0.5) * (sU[l] * sRg[l] + sU[l + 1] * sRg[l + 1]);
+    const T pb = T(0.5) * (sPh[l] + sPh[l + 1]);
+    const T gb = T(0.5) * (sRg[l] + sRg[l + 1]);
+    return fabs(ub) + fsqrt(gb * pb);
+  };
+  auto lam_y = [&](int l) {
+    const T vb = T(0.5) * (sV[l] * sRg[l] + sV[l + EX] * sRg[l + EX]);
+    const T pb = T(0.5) * (sPh[l] + sPh[l + EX]);
+    const T gb = T(0.5) * (sRg[l] + sRg[l + EX]);
+    return fabs(vb) + fsqrt(gb * pb);
+  };
+
+  // ---- phase 2: WENO windows and face shares.  Thread (tx, ty) owns the
+  // windows centred on its own cell along x and along y; its own shares stay
+  // in registers for phase 3.
   const int tx = tid % TX, ty = tid / TX;
+  T fr, fl, er, el, fu, fd, eu, ed;
   {
     const int l = (ty + 3) * EX + tx + 3;
     const int fx = ty * (TX + 1) + tx;   // left x-face of the cell
     const int fy = ty * TX + tx;         // bottom y-face of the cell
-#pragma unroll
-    for (int var = 0; var < 3; ++var) {
-      const T* sv = (var == 0) ? sFx : ((var == 1) ? sE : sPt);
-      const T eps = (var == 0) ? a.epsF : ((var == 1) ? a.epsE : a.epsP);
-      const WenoWin<T> wx = weno_window(sv[l - 2], sv[l - 1], sv[l], sv[l + 1], sv[l + 2], eps);
-      xp[var][fx + 1] = weno_right(wx);
-      xm[var][fx] = weno_left(wx);
-      const T* sw = (var == 0) ? sFy : ((var == 1) ? sE : sPt);
-      const WenoWin<T> wy = weno_window(sw[l - 2 * EX], sw[l - EX], sw[l], sw[l + EX], sw[l + 2 * EX], eps);
-      yp[var][fy + TX] = weno_right(wy);
-      ym[var][fy] = weno_left(wy);
+    {
+      const WenoWin<T> wp = weno_window(sPt[l - 2], sPt[l - 1], sPt[l], sPt[l + 1], sPt[l + 2], a.epsP);
+      const WenoWin<T> wf = weno_window(sFx[l - 2], sFx[l - 1], sFx[l], sFx[l + 1], sFx[l + 2], a.epsF);
+      const WenoWin<T> we = weno_window(sE[l - 2], sE[l - 1], sE[l], sE[l + 1], sE[l + 2], a.epsE);
+      const T lr = lam_x(l), ll = lam_x(l - 1);
+      fr = T(0.5) * (weno_right(wf) + lr * weno_right(wp));
+      fl = T(0.5) * (weno_left(wf) - ll * weno_left(wp));
+      er = T(0.5) * weno_right(we);
+      el = T(0.5) * weno_left(we);
+      xpF[fx + 1] = fr;
+      xmF[fx] = fl;
+      xpE[fx + 1] = er;
+      xmE[fx] = el;
+    }
+    {
+      const WenoWin<T> wp = weno_window(sPt[l - 2 * EX], sPt[l - EX], sPt[l], sPt[l + EX], sPt[l + 2 * EX], a.epsP);
+      const WenoWin<T> wf = weno_window(sFy[l - 2 * EX], sFy[l - EX], sFy[l], sFy[l + EX], sFy[l + 2 * EX], a.epsF);
+      const WenoWin<T> we = weno_window(sE[l - 2 * EX], sE[l - EX], sE[l], sE[l + EX], sE[l + 2 * EX], a.epsE);
+      const T lu = lam_y(l), ld = lam_y(l - EX);
+      fu = T(0.5) * (weno_right(wf) + lu * weno_right(wp));
+      fd = T(0.5) * (weno_left(wf) - ld * weno_left(wp));
+      eu = T(0.5) * weno_right(we);
+      ed = T(0.5) * weno_left(we);
+      ypF[fy + TX] = fu;
+      ymF[fy] = fd;
+      ypE[fy + TX] = eu;
+      ymE[fy] = ed;
     }
   }
   {
-    // extra windows, one uniform code path: x: centres -1 (right face -> x-face
-    // 0) and TX (left face -> x-face TX) of each row; y: centres -1 and TY of
-    // each column.  Each lane gets an LDS base, a stride and an output slot.
-    constexpr int NXE = 2 * TY, NYE = 2 * TX, NPER = NXE + NYE, NEXTRA = 3 * NPER;
-    for (int e = tid; e < NEXTRA; e += NT) {
-      const int var = e / NPER;
-      const int k = e % NPER;
-      int l0, stride, out;
+    // windows centred one cell outside the tile, which give only the share
+    // of the tile's boundary face: x: centres -1 (right face -> x-face 0) and
+    // TX (left face -> x-face TX) of each row; y: centres -1 and TY of each
+    // column.  One lane per window centre, all three variables.
+    constexpr int NXE = 2 * TY, NEXTRA = 2 * TY + 2 * TX;
+    static_assert(NEXTRA <= NT, "one pass of extra windows");
+    if (tid < NEXTRA) {
+      int l0, stride, lf;
       bool left;
-      if (k < NXE) {
-        const int row = k >> 1;
-        left = (k & 1) != 0;
-        l0 = (row + 3) * EX + (left ? TX : -1) + 3 + (var == 0 ? 0 : (var + 1) * NE);
+      T* oF;
+      T* oE;
+      if (tid < NXE) {
+        const int row = tid >> 1;
+        left = (tid & 1) != 0;
+        l0 = (row + 3) * EX + (left ? TX : -1) + 3;
         stride = 1;
-        out = left ? (3 + var) * NFXF + row * (TX + 1) + TX : var * NFXF + row * (TX + 1);
+        const int f = row * (TX + 1) + (left ? TX : 0);
+        oF = left ? xmF + f : xpF + f;
+        oE = left ? xmE + f : xpE + f;
+        lf = left ? l0 - 1 : l0;  // cell left of the face
       } else {
-        const int k2 = k - NXE;
+        const int k2 = tid - NXE;
         const int col = k2 % TX;
         left = k2 >= TX;
-        l0 = ((left ? TY : -1) + 3) * EX + col + 3 + (var == 0 ? NE : (var + 1) * NE);
+        l0 = ((left ? TY : -1) + 3) * EX + col + 3;
         stride = EX;
-        out = 6 * NFXF + (left ? (3 + var) * NFYF + TY * TX + col : var * NFYF + col);
+        const int f = (left ? TY * TX : 0) + col;
+        oF = left ? ymF + f : ypF + f;
+        oE = left ? ymE + f : ypE + f;
+        lf = left ? l0 - EX : l0;  // cell below the face
       }
-      const T eps = (var == 0) ? a.epsF : ((var == 1) ? a.epsE : a.epsP);
-      const WenoWin<T> w = weno_window(sc[l0 - 2 * stride], sc[l0 - stride], sc[l0], sc[l0 + stride],
-                                       sc[l0 + 2 * stride], eps);
-      sfc[out] = weno_side(w, left);
+      const T* sF = (tid < NXE) ? sFx : sFy;
+      const WenoWin<T> wp = weno_window(sPt[l0 - 2 * stride], sPt[l0 - stride], sPt[l0], sPt[l0 + stride],
+                                        sPt[l0 + 2 * stride], a.epsP);
+      const WenoWin<T> wf = weno_window(sF[l0 - 2 * stride], sF[l0 - stride], sF[l0], sF[l0 + stride],
+                                        sF[l0 + 2 * stride], a.epsF);
+      const WenoWin<T> we = weno_window(sE[l0 - 2 * stride], sE[l0 - stride], sE[l0], sE[l0 + stride],
+                                        sE[l0 + 2 * stride], a.epsE);
+      const T lam = (tid < NXE) ? lam_x(lf) : lam_y(lf);
+      const T vp = weno_side(wp, left), vf = weno_side(wf, left), ve = weno_side(we, left);
+      *oF = T(0.5) * (left ? vf - lam * vp : vf + lam * vp);
+      *oE = T(0.5) * ve;
     }
   }
-  __syncthreads();
-
-  // ---- phase 2b: Rusanov fluxes; thread (tx, ty) does the right x-face and
-  // the top y-face of its cell, the tile's left and bottom faces are extra.
-  // Results overwrite xp[0]/xp[1] (x) and yp[0]/yp[1] (y).
-  auto xface = [&](int fi, int row) {
-    const int l = (row + 3) * EX + fi + 2;  // cell left of the face
-    const T ub = T(0.5) * (sUC[l] + sUC[l + 1]);
-    const T pb = T(0.5) * (sPh[l] + sPh[l + 1]);
-    const T gb = T(0.5) * (sRg[l] + sRg[l + 1]);
-    const T lam = fabs(ub) + fsqrt(gb * pb);
-    const int f = row * (TX + 1) + fi;
-    const T F = T(0.5) * (xp[0][f] + xm[0][f] - lam * (xm[2][f] - xp[2][f]));
-    const T E = T(0.5) * (xp[1][f] + xm[1][f]);
-    xp[0][f] = F;
-    xp[1][f] = E;
-  };
-  auto yface = [&](int col, int fj) {
-    const int l = (fj + 2) * EX + col + 3;  // cell below the face
-    const T vb = T(0.5) * (sVC[l] + sVC[l + EX]);
-    const T pb = T(0.5) * (sPh[l] + sPh[l + EX]);
-    const T gb = T(0.5) * (sRg[l] + sRg[l + EX]);
-    const T lam = fabs(vb) + fsqrt(gb * pb);
-    const int f = fj * TX + col;
-    const T F = T(0.5) * (yp[0][f] + ym[0][f] - lam * (ym[2][f] - yp[2][f]));
-    const T E = T(0.5) * (yp[1][f] + ym[1][f]);
-    yp[0][f] = F;
-    yp[1][f] = E;
-  };
-  xface(tx + 1, ty);
-  yface(tx, ty + 1);
-  static_assert(TY <= 64 && TX <= 64, "extra faces: x in wave 0, y in wave 1");
-  if (tid < TY) xface(0, tid);
-  else if (tid >= 64 && tid < 64 + TX) yface(tid - 64, 0);
   __syncthreads();
 
   // ---- phase 3: cells
@@ -383,17 +400,20 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
   const T rdx = a.rdx;
   const int lc = (ty + 3) * EX + (tx + 3);
   const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
-  const T dFx = (xp[0][fx + 1] - xp[0][fx]) * rdx;
-  const T dEx = (xp[1][fx + 1] - xp[1][fx]) * rdx;
-  const T dFy = (yp[0][fy + TX] - yp[0][fy]) * rdx;
-  const T dEy = (yp[1][fy + TX] - yp[1][fy]) * rdx;
+  // face flux = the two shares; both cells beside a face form the same sum
+  // (IEEE addition commutes), so the fluxes telescope and mass is conserved
+  const T dFx = ((fr + xmF[fx + 1]) - (xpF[fx] + fl)) * rdx;
+  const T dEx = ((er + xmE[fx + 1]) - (xpE[fx] + el)) * rdx;
+  const T dFy = ((fu + ymF[fy + TX]) - (ypF[fy] + fd)) * rdx;
+  const T dEy = ((eu + ymE[fy + TX]) - (ypE[fy] + ed)) * rdx;
   const T dvdx = d6f(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], rdx);
   const T dudy = d6f(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX], sU[lc + 3 * EX], rdx);
   const unsigned go = (gt + __umul24(static_cast<unsigned>(ty + 3), P) + static_cast<unsigned>(tx + 3)) * W;
   const T zeta = dvdx - dudy + ldo(st.fs, go);
   const T tp = -(dFx + dFy);
-  const T tu = -dEx + zeta * sVC[lc];
-  const T tv = -dEy - zeta * sUC[lc];
+  const T rgc = sRg[lc];
+  const T tu = -dEx + zeta * (sV[lc] * rgc);
+  const T tv = -dEy - zeta * (sU[lc] * rgc);
   const unsigned un_ = static_cast<unsigned>(n);
   const unsigned io = (static_cast<unsigned>(p) * un_ * un_ + static_cast<unsigned>(j0) * un_ +
                        static_cast<unsigned>(i0) + __umul24(static_cast<unsigned>(ty), un_) +
@@ -533,13 +553,15 @@ void launch_split_ty(hipStream_t s, int kind, const SplitArgs<T>& a, const Split
   }
 }
 
-// Tile heights instantiated; the launch fills nbx, nby for the chosen one.
+// Tile heights instantiated (8, 16, 32); the launch fills nbx, nby for the
+// chosen one.
 template <typename T>
 void launch_split(hipStream_t s, int kind, SplitArgs<T> a, const SplitStatics<T>& st, const HaloScatter& hs,
                   int tile_y) {
   a.nbx = (a.n + TX - 1) / TX;
   a.nby = (a.n + tile_y - 1) / tile_y;
   if (tile_y == 16) launch_split_ty<T, 16>(s, kind, a, st, hs);
+  else if (tile_y == 32) launch_split_ty<T, 32>(s, kind, a, st, hs);
   else launch_split_ty<T, 8>(s, kind, a, st, hs);
 }
 
@@ -551,7 +573,7 @@ struct SplitState {
   HaloScatter hs{};
   bool fp64 = false;
   int n = 0, h = 0;
-  int tile_y = 8;  // stage-kernel tile height (8 or 16)
+  int tile_y = 16;  // stage-kernel tile height (8, 16 or 32)
   std::size_t np = 0, ni = 0, esz = 4;
   void* dmem = nullptr;
   // typed device arrays (float or double)
@@ -673,8 +695,8 @@ int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out,
   }
   if (const char* ty = std::getenv("CCS_SPLIT_TILE_Y")) {
     const int v = std::atoi(ty);
-    if (v != 8 && v != 16) {
-      *err = "CCS_SPLIT_TILE_Y must be 8 or 16";
+    if (v != 8 && v != 16 && v != 32) {
+      *err = "CCS_SPLIT_TILE_Y must be 8, 16 or 32";
       delete s;
       return CCS_ERR_INVALID;
     }
