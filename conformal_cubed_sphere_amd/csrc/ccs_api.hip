@@ -45,14 +45,20 @@ struct ccs_handle {
   double* result = nullptr;
   // split-precision path
   SplitState* split = nullptr;
-  // graph of one step
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  // graphs of one step ([0]) and of kGraphSteps steps ([1]); every step
+  // leaves the state in the same buffer, so a graph of several steps is the
+  // one-step sequence repeated
+  hipGraph_t graph[2]{};
+  hipGraphExec_t graph_exec[2]{};
 };
 
 namespace {
 
 const char* kVersion = "ccs-mi355x 0.1.0 (gfx950)";
+
+// Steps per replay of the long graph: one launch per kGraphSteps steps keeps
+// the host submission rate far above what the GPU consumes.
+constexpr int kGraphSteps = 16;
 
 int fail(ccs_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -204,8 +210,10 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
 
 extern "C" void ccs_destroy(ccs_handle* hd) {
   if (!hd) return;
-  if (hd->graph_exec) hipGraphExecDestroy(hd->graph_exec);
-  if (hd->graph) hipGraphDestroy(hd->graph);
+  for (int g = 0; g < 2; ++g) {
+    if (hd->graph_exec[g]) hipGraphExecDestroy(hd->graph_exec[g]);
+    if (hd->graph[g]) hipGraphDestroy(hd->graph[g]);
+  }
   if (hd->stream) hipStreamSynchronize(hd->stream);
   if (hd->dmem) hipFree(hd->dmem);
   if (hd->split) split_destroy(hd->split);
@@ -307,17 +315,21 @@ extern "C" int ccs_step(ccs_handle* hd, int nsteps) {
     k = 1;
   }
   if (k < nsteps && hd->cfg.use_graph) {
-    if (!hd->graph_exec) {
+    const int want = (nsteps - k >= kGraphSteps) ? 1 : 0;
+    for (int g = 0; g <= want; ++g) {
+      if (hd->graph_exec[g]) continue;
       HIPCHK(hd, hipStreamBeginCapture(hd->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_step(hd, false, nullptr);
-      hipGraph_t g = nullptr;
-      const hipError_t ec = hipStreamEndCapture(hd->stream, &g);
+      int rc = CCS_OK;
+      for (int s = 0; s < (g ? kGraphSteps : 1) && rc == CCS_OK; ++s) rc = enqueue_step(hd, false, nullptr);
+      hipGraph_t gr = nullptr;
+      const hipError_t ec = hipStreamEndCapture(hd->stream, &gr);
       if (rc != CCS_OK) return rc;
       if (ec != hipSuccess) return fail(hd, CCS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-      hd->graph = g;
-      HIPCHK(hd, hipGraphInstantiate(&hd->graph_exec, hd->graph, nullptr, nullptr, 0));
+      hd->graph[g] = gr;
+      HIPCHK(hd, hipGraphInstantiate(&hd->graph_exec[g], hd->graph[g], nullptr, nullptr, 0));
     }
-    for (; k < nsteps; ++k) HIPCHK(hd, hipGraphLaunch(hd->graph_exec, hd->stream));
+    for (; k + kGraphSteps <= nsteps; k += kGraphSteps) HIPCHK(hd, hipGraphLaunch(hd->graph_exec[1], hd->stream));
+    for (; k < nsteps; ++k) HIPCHK(hd, hipGraphLaunch(hd->graph_exec[0], hd->stream));
   }
   for (; k < nsteps; ++k) {
     const int rc = enqueue_step(hd, false, nullptr);

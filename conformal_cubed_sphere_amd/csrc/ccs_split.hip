@@ -212,17 +212,17 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
   constexpr int EY = TY + 6;
   constexpr int NE = EX * EY;
   constexpr unsigned W = sizeof(T);
-  // cell fields on the extended tile: 0 Fx, 1 Fy, 2 E, 3 phitG, 4 phit,
-  // 5 1/sqrtg, 6 U, 7 V
-  __shared__ T sc[8 * NE];
+  // cell fields on the extended tile: 0 Fx, 1 Fy, 2 E, 3 phitG, 4 1/sqrtg,
+  // 5 U, 6 V (phit = phitG / sqrtg is formed where needed; seven fields keep
+  // a 16-row tile within a quarter of the CU's LDS)
+  __shared__ T sc[7 * NE];
   T* const sFx = sc;
   T* const sFy = sc + NE;
   T* const sE = sc + 2 * NE;
   T* const sPt = sc + 3 * NE;
-  T* const sPh = sc + 4 * NE;
-  T* const sRg = sc + 5 * NE;
-  T* const sU = sc + 6 * NE;
-  T* const sV = sc + 7 * NE;
+  T* const sRg = sc + 4 * NE;
+  T* const sU = sc + 5 * NE;
+  T* const sV = sc + 6 * NE;
   // face shares: x-faces [TY][TX+1], y-faces [TY+1][TX]; per face the share
   // of the flux of phitG*u (F) and of E from its left/lower side (p) and its
   // right/upper side (m)
@@ -286,7 +286,6 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
     sFx[l] = Pv * Uc;
     sFy[l] = Pv * Vc;
     sPt[l] = phit * sg;
-    sPh[l] = phit;
     sRg[l] = rg;
     sU[l] = U;
     sV[l] = V;
@@ -297,15 +296,17 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
   // at the y-face between l and l + EX (spatial_operators_mod.F90:109-116,
   // 134-141, on a conformal grid)
   auto lam_x = [&](int l) {
-    const T ub = T(0.5) * (sU[l] * sRg[l] + sU[l + 1] * sRg[l + 1]);
-    const T pb = T(0.5) * (sPh[l] + sPh[l + 1]);
-    const T gb = T(0.5) * (sRg[l] + sRg[l + 1]);
+    const T r0 = sRg[l], r1 = sRg[l + 1];
+    const T ub = T(0.5) * (sU[l] * r0 + sU[l + 1] * r1);
+    const T pb = T(0.5) * (sPt[l] * r0 + sPt[l + 1] * r1);
+    const T gb = T(0.5) * (r0 + r1);
     return fabs(ub) + fsqrt(gb * pb);
   };
   auto lam_y = [&](int l) {
-    const T vb = T(0.5) * (sV[l] * sRg[l] + sV[l + EX] * sRg[l + EX]);
-    const T pb = T(0.5) * (sPh[l] + sPh[l + EX]);
-    const T gb = T(0.5) * (sRg[l] + sRg[l + EX]);
+    const T r0 = sRg[l], r1 = sRg[l + EX];
+    const T vb = T(0.5) * (sV[l] * r0 + sV[l + EX] * r1);
+    const T pb = T(0.5) * (sPt[l] * r0 + sPt[l + EX] * r1);
+    const T gb = T(0.5) * (r0 + r1);
     return fabs(vb) + fsqrt(gb * pb);
   };
 

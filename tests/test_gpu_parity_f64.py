@@ -75,19 +75,24 @@ def test_rk4_steps_match_oracle(ccs, orc):
         assert err <= 1e-13
 
 
-def test_graph_replay_equals_eager(ccs):
+@pytest.mark.parametrize("precision,scheme", [("fp64", "RK3_TVD"), ("fp64", "RK4"), ("fp32", "RK3_TVD")])
+def test_graph_replay_equals_eager(ccs, precision, scheme):
+    """One-step and 16-step graphs, replayed several times with host work and
+    diagnostics between the calls, give the eager launches' state bit for bit."""
     mesh, s0, dt, _ = _setup(ccs, 48, 3, 7, True, seed=4)
     out = []
     for use_graph in (False, True):
-        print(f"graph={use_graph}: 6 steps in 3 calls", flush=True)
-        with ccs.DynamicalCore(mesh, dt, use_graph=use_graph) as core:
+        print(f"{precision} {scheme} graph={use_graph}: 43 steps in 5 calls", flush=True)
+        with ccs.DynamicalCore(mesh, dt, precision=precision, scheme=scheme, use_graph=use_graph) as core:
             core.set_state(s0)
-            core.step(1)
-            core.step(2)
-            core.step(3)
-            out.append(core.get_state())
+            diag = []
+            for n in (1, 2, 3, 17, 20):
+                core.step(n)
+                diag.append(core.diagnostics())
+            out.append((core.get_state(), diag))
     for k in ("phiG", "u", "v"):
-        assert np.array_equal(getattr(out[0], k), getattr(out[1], k)), k
+        assert np.array_equal(getattr(out[0][0], k), getattr(out[1][0], k)), k
+    assert out[0][1] == out[1][1]
 
 
 def test_diagnostics_compensated(ccs, orc):
