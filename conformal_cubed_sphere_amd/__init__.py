@@ -5,11 +5,13 @@ Host-side mirror of the reference dynamic core's operator interface
 tables of ``read_mesh`` (io_mod.F90), the initial states of ``initTestCase``
 (test_case_mod.F90), and the time-step procedures ``RK3_TVD`` / ``RK4``
 (time_scheme_mod.F90), ``spatial_operator`` (spatial_operators_mod.F90) and
-the diagnostics of diag_mod.F90 — all running on the GPU through libccs.so.
+the diagnostics of diag_mod.F90 — all running on the GPU through libccs.so —
+and a batched ensemble of independent runs (Ensemble).
 """
 from .core import (  # noqa: F401
     CcsError,
     DynamicalCore,
+    Ensemble,
     Mesh,
     State,
     cfl_time_step,
@@ -17,4 +19,4 @@ from .core import (  # noqa: F401
     init_case,
 )
 
-__all__ = ["CcsError", "DynamicalCore", "Mesh", "State", "cfl_time_step", "error_norms", "init_case"]
+__all__ = ["CcsError", "DynamicalCore", "Ensemble", "Mesh", "State", "cfl_time_step", "error_norms", "init_case"]

@@ -65,6 +65,17 @@ class CcsConfig(ctypes.Structure):
     ]
 
 
+class CcsEnsembleConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_members", ctypes.c_int), ("halo", ctypes.c_int), ("precision", ctypes.c_int),
+        ("scheme", ctypes.c_int), ("device", ctypes.c_int), ("use_graph", ctypes.c_int),
+        ("tile_y", ctypes.c_int), ("persistent", ctypes.c_int),
+        ("radius", ctypes.c_double), ("gravity", ctypes.c_double), ("omega", ctypes.c_double),
+    ]
+
+
+_I = ctypes.POINTER(ctypes.c_int)
+
 # Every entry point declared in include/ccs.h, with its ctypes signature.
 SIGNATURES = {
     "ccs_mesh_geometry": (ctypes.c_int, [ctypes.c_int, _D, _D, _D, _D]),
@@ -84,6 +95,18 @@ SIGNATURES = {
     "ccs_destroy": (None, [ctypes.c_void_p]),
     "ccs_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "ccs_version": (ctypes.c_char_p, []),
+    "ccs_ensemble_create": (ctypes.c_int, [ctypes.POINTER(CcsEnsembleConfig), _I, _D, _D,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "ccs_ensemble_set_mesh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D]),
+    "ccs_ensemble_set_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D]),
+    "ccs_ensemble_get_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D]),
+    "ccs_ensemble_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D]),
+    "ccs_ensemble_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ccs_ensemble_autotune": (ctypes.c_int, [ctypes.c_void_p, _I, _I, _D]),
+    "ccs_ensemble_time_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "ccs_ensemble_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccs_ensemble_destroy": (None, [ctypes.c_void_p]),
+    "ccs_ensemble_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 
 

@@ -253,3 +253,97 @@ class DynamicalCore:
         _check("ccs_time_steps", lib.ccs_time_steps(self._h, int(nsteps), ctypes.byref(total),
                                                     ctypes.byref(kern) if kernel_timing else None), self._h)
         return total.value, (kern.value if kernel_timing else None)
+
+
+class Ensemble:
+    """Many independent runs advanced together (the ccs_ensemble_* entry
+    points): one kernel launch per RK stage covers every member's tiles.
+    Members have their own mesh size, initial state and time step; each
+    evolves exactly as a DynamicalCore with the same inputs would.
+
+    tile_y: 8, 16, 32, or 0 to let the autotuner choose; persistent: 0, 1, or
+    -1 to let the autotuner choose.
+    """
+
+    def __init__(self, meshes, dts, precision: str = "fp32", scheme: str = "RK3_TVD", device: int = 0,
+                 use_graph: bool = True, tile_y: int = 0, persistent: int = -1, gravity: float = GRAVITY):
+        meshes = list(meshes)
+        if not meshes or len(meshes) != len(dts):
+            raise ValueError("one time step per member")
+        halo = meshes[0].halo
+        if any(m.halo != halo or m.radius != meshes[0].radius or m.omega != meshes[0].omega for m in meshes):
+            raise ValueError("members share halo, radius and omega")
+        cfg = _lib.CcsEnsembleConfig()
+        cfg.n_members, cfg.halo = len(meshes), halo
+        cfg.precision, cfg.scheme = PRECISIONS[precision], SCHEMES[scheme]
+        cfg.device, cfg.use_graph, cfg.tile_y, cfg.persistent = int(device), int(bool(use_graph)), int(tile_y), \
+            int(persistent)
+        cfg.radius, cfg.gravity, cfg.omega = meshes[0].radius, gravity, meshes[0].omega
+        n = (ctypes.c_int * len(meshes))(*[m.n for m in meshes])
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        dx = np.array([m.dx for m in meshes], dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check("ccs_ensemble_create", lib.ccs_ensemble_create(ctypes.byref(cfg), n, dptr(dt), dptr(dx),
+                                                              ctypes.byref(h)))
+        self._h = h
+        self.meshes, self.dts, self.precision, self.scheme = meshes, dt, precision, scheme
+        self._phis = [None] * len(meshes)
+
+    def _chk(self, where, code):
+        if code != _lib.CCS_OK:
+            raise CcsError(where, code, lib.ccs_ensemble_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ccs_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.meshes)
+
+    def set_state(self, member: int, s: State):
+        m = self.meshes[member]
+        self._phis[member] = np.ascontiguousarray(s.phis, dtype=np.float64)
+        self._chk("ccs_ensemble_set_mesh", lib.ccs_ensemble_set_mesh(
+            self._h, member, dptr(m.sqrtG), dptr(m.G), dptr(m.IG), dptr(m.A), dptr(m.IA), dptr(m.f),
+            dptr(self._phis[member])))
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (s.phiG, s.u, s.v)]
+        self._chk("ccs_ensemble_set_state", lib.ccs_ensemble_set_state(self._h, member, *(dptr(a) for a in args)))
+
+    def autotune(self):
+        ty, ps, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._chk("ccs_ensemble_autotune", lib.ccs_ensemble_autotune(self._h, ctypes.byref(ty), ctypes.byref(ps),
+                                                                     ctypes.byref(ms)))
+        return {"tile_y": ty.value, "persistent": ps.value, "stage_ms": ms.value}
+
+    def step(self, nsteps: int = 1):
+        self._chk("ccs_ensemble_step", lib.ccs_ensemble_step(self._h, int(nsteps)))
+
+    def synchronize(self):
+        self._chk("ccs_ensemble_synchronize", lib.ccs_ensemble_synchronize(self._h))
+
+    def time_steps(self, nsteps: int) -> float:
+        total = ctypes.c_double()
+        self._chk("ccs_ensemble_time_steps", lib.ccs_ensemble_time_steps(self._h, int(nsteps), ctypes.byref(total)))
+        return total.value
+
+    def get_state(self, member: int) -> State:
+        a = {k: np.zeros(self.meshes[member].npad) for k in ("phiG", "u", "v", "phi", "uc", "vc", "zonal", "merid")}
+        self._chk("ccs_ensemble_get_state", lib.ccs_ensemble_get_state(
+            self._h, member, *(dptr(a[k]) for k in ("phiG", "u", "v", "phi", "uc", "vc", "zonal", "merid"))))
+        return State(phis=self._phis[member], **a)
+
+    def diagnostics(self, member: int):
+        mass, energy = ctypes.c_double(), ctypes.c_double()
+        self._chk("ccs_ensemble_diagnostics", lib.ccs_ensemble_diagnostics(self._h, member, ctypes.byref(mass),
+                                                                           ctypes.byref(energy)))
+        return mass.value, energy.value

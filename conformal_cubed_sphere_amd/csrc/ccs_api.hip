@@ -461,3 +461,278 @@ extern "C" int ccs_time_steps(ccs_handle* hd, int nsteps, double* total_ms, doub
   hipEventDestroy(e1);
   return rc;
 }
+
+// ------------------------------------------------------------------ ensemble
+// Members are ordinary split-path handles (own buffers, statics, dt); the
+// ensemble owns the batched launch plan, a stream and the step graphs.
+struct ccs_ensemble {
+  ccs_ensemble_config cfg{};
+  std::vector<ccs_handle*> m;
+  hipStream_t stream = nullptr;
+  std::string err;
+  EnsPlan* plan = nullptr;
+  int tile_y = 0, persistent = 0;
+  double stage_ms = 0.0;
+  hipGraph_t graph[2]{};
+  hipGraphExec_t graph_exec[2]{};
+};
+
+namespace {
+
+int efail(ccs_ensemble* e, int code, const std::string& msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+#define ENSCHK(e, call)                                                                       \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return efail((e), CCS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+void ens_drop_graphs(ccs_ensemble* e) {
+  for (int g = 0; g < 2; ++g) {
+    if (e->graph_exec[g]) (void)hipGraphExecDestroy(e->graph_exec[g]);
+    if (e->graph[g]) (void)hipGraphDestroy(e->graph[g]);
+    e->graph_exec[g] = nullptr;
+    e->graph[g] = nullptr;
+  }
+}
+
+int ens_member(ccs_ensemble* e, int member) {
+  if (!e) return CCS_ERR_INVALID;
+  if (member < 0 || member >= static_cast<int>(e->m.size()))
+    return efail(e, CCS_ERR_INVALID, "ensemble: member index out of range");
+  return CCS_OK;
+}
+
+// pass a member's error text up to the ensemble
+int ens_forward(ccs_ensemble* e, int member, int rc) {
+  if (rc != CCS_OK) e->err = "member " + std::to_string(member) + ": " + e->m[member]->err;
+  return rc;
+}
+
+std::vector<SplitState*> ens_states(const ccs_ensemble* e) {
+  std::vector<SplitState*> s;
+  for (ccs_handle* h : e->m) s.push_back(h->split);
+  return s;
+}
+
+}  // namespace
+
+extern "C" const char* ccs_ensemble_last_error(const ccs_ensemble* e) {
+  if (!e) return "null ensemble";
+  return e->err.c_str();
+}
+
+extern "C" int ccs_ensemble_create(const ccs_ensemble_config* cfg, const int* n, const double* dt, const double* dx,
+                                   ccs_ensemble** out) {
+  if (!cfg || !n || !dt || !dx || !out) return CCS_ERR_INVALID;
+  *out = nullptr;
+  if (cfg->n_members < 1 || cfg->halo < 3) return CCS_ERR_INVALID;
+  if (cfg->precision != CCS_FP32_SPLIT && cfg->precision != CCS_FP64_FAST) return CCS_ERR_INVALID;
+  // RK4 ensembles are switched off: their first GPU run faulted and the
+  // cause is not yet found (the single-run RK4 path is unaffected)
+  if (cfg->scheme != CCS_RK3_TVD) return CCS_ERR_INVALID;
+  if (cfg->tile_y != 0 && cfg->tile_y != 8 && cfg->tile_y != 16 && cfg->tile_y != 32) return CCS_ERR_INVALID;
+  if (cfg->persistent < -1 || cfg->persistent > 1) return CCS_ERR_INVALID;
+  for (int k = 0; k < cfg->n_members; ++k)
+    if (n[k] < 4 || n[k] < cfg->halo || !(dt[k] > 0.0) || !(dx[k] > 0.0)) return CCS_ERR_INVALID;
+  ccs_ensemble* e = new ccs_ensemble();
+  e->cfg = *cfg;
+  for (int k = 0; k < cfg->n_members; ++k) {
+    ccs_config c{};
+    c.n = n[k];
+    c.halo = cfg->halo;
+    c.precision = cfg->precision;
+    c.scheme = cfg->scheme;
+    c.eflux = CCS_EFLUX_NONE;
+    c.device = cfg->device;
+    c.use_graph = 0;
+    c.dt = dt[k];
+    c.dx = dx[k];
+    c.radius = cfg->radius;
+    c.gravity = cfg->gravity;
+    c.omega = cfg->omega;
+    ccs_handle* h = nullptr;
+    const int rc = ccs_create(&c, &h);
+    if (rc != CCS_OK) {
+      ccs_ensemble_destroy(e);
+      return rc;
+    }
+    e->m.push_back(h);
+  }
+  if (hipSetDevice(cfg->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    ccs_ensemble_destroy(e);
+    return CCS_ERR_DEVICE;
+  }
+  *out = e;
+  return CCS_OK;
+}
+
+extern "C" void ccs_ensemble_destroy(ccs_ensemble* e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_drop_graphs(e);
+  if (e->plan) ens_plan_destroy(e->plan);
+  for (ccs_handle* h : e->m) ccs_destroy(h);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+extern "C" int ccs_ensemble_set_mesh(ccs_ensemble* e, int member, const double* sqrtG, const double* G,
+                                     const double* IG, const double* A, const double* IA, const double* f,
+                                     const double* phis) {
+  int rc = ens_member(e, member);
+  if (rc != CCS_OK) return rc;
+  ENSCHK(e, hipStreamSynchronize(e->stream));
+  return ens_forward(e, member, ccs_set_mesh(e->m[member], sqrtG, G, IG, A, IA, f, phis));
+}
+
+extern "C" int ccs_ensemble_set_state(ccs_ensemble* e, int member, const double* phiG, const double* u,
+                                      const double* v) {
+  int rc = ens_member(e, member);
+  if (rc != CCS_OK) return rc;
+  ENSCHK(e, hipStreamSynchronize(e->stream));
+  return ens_forward(e, member, ccs_set_state(e->m[member], phiG, u, v, nullptr, nullptr, nullptr));
+}
+
+extern "C" int ccs_ensemble_get_state(ccs_ensemble* e, int member, double* phiG, double* u, double* v, double* phi,
+                                      double* uc, double* vc, double* zonal, double* merid) {
+  int rc = ens_member(e, member);
+  if (rc != CCS_OK) return rc;
+  ENSCHK(e, hipStreamSynchronize(e->stream));
+  return ens_forward(e, member, ccs_get_state(e->m[member], phiG, u, v, phi, uc, vc, zonal, merid));
+}
+
+extern "C" int ccs_ensemble_diagnostics(ccs_ensemble* e, int member, double* mass, double* energy) {
+  int rc = ens_member(e, member);
+  if (rc != CCS_OK) return rc;
+  ENSCHK(e, hipStreamSynchronize(e->stream));
+  return ens_forward(e, member, ccs_diagnostics(e->m[member], mass, energy));
+}
+
+extern "C" int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persistent, double* stage_ms) {
+  if (!e) return CCS_ERR_INVALID;
+  for (ccs_handle* h : e->m)
+    if (!h->mesh_set) return efail(e, CCS_ERR_ORDER, "ccs_ensemble_autotune before every member's mesh is set");
+  ENSCHK(e, hipSetDevice(e->cfg.device));
+  ENSCHK(e, hipStreamSynchronize(e->stream));
+  std::vector<SplitState*> st = ens_states(e);
+  std::vector<int> tys, pers;
+  if (e->cfg.tile_y) tys.push_back(e->cfg.tile_y); else tys = {8, 16, 32};
+  if (e->cfg.persistent >= 0) pers.push_back(e->cfg.persistent); else pers = {0, 1};
+  hipEvent_t e0, e1;
+  ENSCHK(e, hipEventCreate(&e0));
+  ENSCHK(e, hipEventCreate(&e1));
+  EnsPlan* best = nullptr;
+  double best_ms = 0.0;
+  int rc = CCS_OK;
+  constexpr int kReps = 6;
+  for (int ty : tys) {
+    for (int ps : pers) {
+      EnsPlan* p = nullptr;
+      rc = ens_plan_create(st.data(), static_cast<int>(st.size()), ty, ps, e->cfg.device, &p, &e->err);
+      if (rc != CCS_OK) break;
+      // the tendency launch has the cost of a stage and changes no state
+      ens_enqueue_tendency(p, e->stream);
+      (void)hipEventRecord(e0, e->stream);
+      for (int r = 0; r < kReps; ++r) ens_enqueue_tendency(p, e->stream);
+      (void)hipEventRecord(e1, e->stream);
+      float ms = 0.f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        ens_plan_destroy(p);
+        rc = efail(e, CCS_ERR_DEVICE, "ccs_ensemble_autotune: timing failed");
+        break;
+      }
+      const double mean = ms / kReps;
+      if (!best || mean < best_ms) {
+        if (best) ens_plan_destroy(best);
+        best = p;
+        best_ms = mean;
+      } else {
+        ens_plan_destroy(p);
+      }
+    }
+    if (rc != CCS_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != CCS_OK) {
+    if (best) ens_plan_destroy(best);
+    return rc;
+  }
+  ens_drop_graphs(e);
+  if (e->plan) ens_plan_destroy(e->plan);
+  e->plan = best;
+  e->stage_ms = best_ms;
+  e->tile_y = ens_plan_tile_y(best);
+  e->persistent = ens_plan_persistent(best);
+  if (tile_y) *tile_y = e->tile_y;
+  if (persistent) *persistent = e->persistent;
+  if (stage_ms) *stage_ms = e->stage_ms;
+  return CCS_OK;
+}
+
+extern "C" int ccs_ensemble_step(ccs_ensemble* e, int nsteps) {
+  if (!e) return CCS_ERR_INVALID;
+  if (nsteps < 0) return efail(e, CCS_ERR_INVALID, "ccs_ensemble_step: nsteps < 0");
+  for (ccs_handle* h : e->m)
+    if (!h->mesh_set || !h->state_set)
+      return efail(e, CCS_ERR_ORDER, "ccs_ensemble_step before every member's mesh and state are set");
+  if (!e->plan) {
+    const int rc = ccs_ensemble_autotune(e, nullptr, nullptr, nullptr);
+    if (rc != CCS_OK) return rc;
+  }
+  ENSCHK(e, hipSetDevice(e->cfg.device));
+  // members' own streams (uploads) are idle: every member call synchronises
+  const int scheme = e->cfg.scheme;
+  int k = 0;
+  if (e->cfg.use_graph && nsteps > 0) {
+    const int want = (nsteps >= kGraphSteps) ? 1 : 0;
+    for (int g = 0; g <= want; ++g) {
+      if (e->graph_exec[g]) continue;
+      ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < (g ? kGraphSteps : 1); ++s) ens_enqueue_step(e->plan, e->stream, scheme);
+      hipGraph_t gr = nullptr;
+      ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
+      e->graph[g] = gr;
+      ENSCHK(e, hipGraphInstantiate(&e->graph_exec[g], e->graph[g], nullptr, nullptr, 0));
+    }
+    for (; k + kGraphSteps <= nsteps; k += kGraphSteps) ENSCHK(e, hipGraphLaunch(e->graph_exec[1], e->stream));
+    for (; k < nsteps; ++k) ENSCHK(e, hipGraphLaunch(e->graph_exec[0], e->stream));
+  }
+  for (; k < nsteps; ++k) ens_enqueue_step(e->plan, e->stream, scheme);
+  ENSCHK(e, hipGetLastError());
+  return CCS_OK;
+}
+
+extern "C" int ccs_ensemble_time_steps(ccs_ensemble* e, int nsteps, double* total_ms) {
+  if (!e) return CCS_ERR_INVALID;
+  if (nsteps <= 0) return efail(e, CCS_ERR_INVALID, "ccs_ensemble_time_steps: nsteps must be positive");
+  if (!e->plan) {
+    const int rc = ccs_ensemble_autotune(e, nullptr, nullptr, nullptr);
+    if (rc != CCS_OK) return rc;
+  }
+  hipEvent_t e0, e1;
+  ENSCHK(e, hipEventCreate(&e0));
+  ENSCHK(e, hipEventCreate(&e1));
+  ENSCHK(e, hipEventRecord(e0, e->stream));
+  const int rc = ccs_ensemble_step(e, nsteps);
+  ENSCHK(e, hipEventRecord(e1, e->stream));
+  ENSCHK(e, hipEventSynchronize(e1));
+  float ms = 0.f;
+  ENSCHK(e, hipEventElapsedTime(&ms, e0, e1));
+  if (total_ms) *total_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+extern "C" int ccs_ensemble_synchronize(ccs_ensemble* e) {
+  if (!e) return CCS_ERR_INVALID;
+  ENSCHK(e, hipStreamSynchronize(e->stream));
+  return CCS_OK;
+}

@@ -28,6 +28,21 @@ int split_tendency(SplitState* s, hipStream_t st, bool first, double* tp, double
                    std::string* err);
 int split_diagnostics(SplitState* s, hipStream_t st, bool first, double* result, std::string* err);
 
+// Batched ensemble of split-path members: one launch per RK stage covers the
+// tiles of every member (tile height 8, 16 or 32; persistent: a grid of
+// CUs x resident workgroups that loops over the tiles, else one workgroup
+// per tile).  Members keep their own buffers, statics and time step.
+struct EnsPlan;
+int ens_plan_create(SplitState* const* mem, int nm, int tile_y, int persistent, int device, EnsPlan** out,
+                    std::string* err);
+void ens_plan_destroy(EnsPlan* e);
+void ens_enqueue_step(const EnsPlan* e, hipStream_t st, int scheme);
+void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st);
+int ens_plan_grid(const EnsPlan* e);
+int ens_plan_tile_y(const EnsPlan* e);
+int ens_plan_persistent(const EnsPlan* e);
+int ens_plan_tiles(const EnsPlan* e);
+
 }  // namespace ccs
 
 #endif

@@ -38,6 +38,7 @@
 //            permutation of the panel-basis change (ghost_mod.f90:100-153).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -205,8 +206,21 @@ __device__ __forceinline__ T d6f(T m3, T m2, T m1, T p1, T p2, T p3, T rdx) {
   return ((p3 - m3) - T(9) * (p2 - m2) + T(45) * (p1 - m1)) * (rdx * (T(1) / T(60)));
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs, each with its own L2, so workgroup b runs on XCD b % 8.  Each XCD is
+// given a contiguous run of tiles (x fastest, then y, then panel, then
+// member) so the border rows and columns that neighbouring tiles share are
+// fetched into one L2 instead of up to three.
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+  const unsigned xq = nb >> 3, xr = nb & 7u, xcd = b & 7u;
+  return xcd * xq + min(xcd, xr) + (b >> 3);
+}
+
+// One RK stage on one 32 x TY tile of panel p with lower-left interior cell
+// (i0, j0); hr = the panel's four halo-scatter rules.
 template <typename T, int KIND, int TY>
-__global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
+__device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitStatics<T>& st, const int (*hr)[11],
+                                           const int p, const int i0, const int j0, const int tid) {
 #pragma clang fp contract(fast)
   constexpr int NT = TX * TY;
   constexpr int EY = TY + 6;
@@ -238,22 +252,8 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
   T* const ypE = ypF + 2 * NFYF;
   T* const ymE = ypF + 3 * NFYF;
 
-  // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
-  // XCDs, each with its own L2, so workgroup b runs on XCD b % 8.  Each XCD
-  // is given a contiguous run of tiles (x fastest, then y, then panel) so the
-  // border rows and columns that neighbouring tiles share are fetched into
-  // one L2 instead of up to three.
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned xq = nb >> 3, xr = nb & 7u, xcd = b & 7u;
-  const unsigned t = xcd * xq + min(xcd, xr) + (b >> 3);
-  const unsigned tq = t / static_cast<unsigned>(a.nbx);
-  const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
-  const int i0 = static_cast<int>(t - tq * a.nbx) * TX;
-  const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
-
   const int n = a.n, h = a.h;
   const unsigned P = static_cast<unsigned>(n + 2 * h);
-  const int tid = threadIdx.x;
   // padded index of the extended tile's (0, 0) cell (i0 - 3, j0 - 3); h >= 3
   const unsigned gt = static_cast<unsigned>(p) * P * P + static_cast<unsigned>(j0 - 3 + h) * P +
                       static_cast<unsigned>(i0 - 3 + h);
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (!strip[s]) continue;
-    const int* r = hs.r[p][s];
+    const int* r = hr[s];
     // halo destination (1-based cell id, jd of panel r[0]); the coefficients
     // r[2], r[3], r[5], r[6] are 0 or +-1
     const int id = r[1] + __mul24(r[2], i1) + __mul24(r[3], j1);
@@ -476,6 +476,82 @@ __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStat
     sto(a.out.phiG, gd, pn);
     sto(a.out.u, gd, static_cast<T>(r[8]) * c[r[7]]);
     sto(a.out.v, gd, static_cast<T>(r[10]) * c[r[9]]);
+  }
+}
+
+template <typename T, int KIND, int TY>
+__global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
+  const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned tq = t / static_cast<unsigned>(a.nbx);
+  const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
+  const int i0 = static_cast<int>(t - tq * a.nbx) * TX;
+  const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
+  stage_tile<T, KIND, TY>(a, st, hs.r[p], p, i0, j0, static_cast<int>(threadIdx.x));
+}
+
+// ---- batched ensemble: every member's tiles in one launch per RK stage.
+// A member's buffers, constants and halo rules sit in a device table; the
+// stage selects which of the three state buffers are input, base and output
+// and which tendency slots it writes (the same for every member).
+template <typename T>
+struct EnsMember {
+  SplitArgs<T> a;      // n, h, nbx, nby, rdx, dts, eps (per member)
+  SplitStatics<T> st;
+  T* buf[3][3];        // [buffer][dP, U, V]
+  T* tend[9];
+  T inc_half, inc_full;  // RK4 increments dt/(2a), dt/a
+  int tile0;             // first global tile of the member
+  int hs[6][4][11];
+};
+
+struct EnsStage {
+  int in_b, base_b, out_b;
+  int tend0;    // first tendency slot written (RK4 stages a)
+  int inc_full; // RK4: 1 -> dt/a increment, 0 -> dt/(2a)
+};
+
+template <typename T, int KIND, int TY>
+__device__ __forceinline__ void ens_tile(const EnsMember<T>* __restrict__ md, const int* __restrict__ tile_member,
+                                         const EnsStage& sel, const unsigned t, const int tid) {
+  const int m = tile_member[t];
+  const EnsMember<T>* M = md + m;
+  SplitArgs<T> a = M->a;
+  a.in = Prog<const T>{M->buf[sel.in_b][0], M->buf[sel.in_b][1], M->buf[sel.in_b][2]};
+  a.base = Prog<const T>{M->buf[sel.base_b][0], M->buf[sel.base_b][1], M->buf[sel.base_b][2]};
+  a.out = Prog<T>{M->buf[sel.out_b][0], M->buf[sel.out_b][1], M->buf[sel.out_b][2]};
+  for (int k = 0; k < 3; ++k) a.tend[k] = M->tend[sel.tend0 + k];
+  for (int k = 0; k < 9; ++k) a.tacc[k] = M->tend[k];
+  a.incs = sel.inc_full ? M->inc_full : M->inc_half;
+  const unsigned lt = t - static_cast<unsigned>(M->tile0);
+  const unsigned tq = lt / static_cast<unsigned>(a.nbx);
+  const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
+  const int i0 = static_cast<int>(lt - tq * a.nbx) * TX;
+  const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
+  stage_tile<T, KIND, TY>(a, M->st, M->hs[p], p, i0, j0, tid);
+}
+
+// PERSIST = false: one workgroup per tile, in the XCD-aware order.
+// PERSIST = true: a grid of resident workgroups; each XCD's workgroups take
+// turns through that XCD's contiguous share of the global tile list.
+template <typename T, int KIND, int TY, bool PERSIST>
+__global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restrict__ md,
+                                                      const int* __restrict__ tile_member, int ntiles, EnsStage sel) {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  if (!PERSIST) {
+    ens_tile<T, KIND, TY>(md, tile_member, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x));
+    return;
+  }
+  const unsigned xcd = b & 7u;
+  const unsigned per = (nb >> 3) + (xcd < (nb & 7u) ? 1u : 0u);  // workgroups on this XCD
+  const unsigned lo = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * xcd) >> 3);
+  const unsigned hi = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * (xcd + 1)) >> 3);
+  for (unsigned t = lo + (b >> 3); t < hi; t += per) {
+    __syncthreads();  // the previous tile's LDS reads are done
+    // keep the thread-index arithmetic inside the loop: hoisting it out would
+    // hold ~30 more VGPRs for the whole kernel and halve the occupancy
+    int tid = static_cast<int>(threadIdx.x);
+    asm volatile("" : "+v"(tid));
+    ens_tile<T, KIND, TY>(md, tile_member, sel, t, tid);
   }
 }
 
@@ -911,5 +987,165 @@ int split_diagnostics(SplitState* s, hipStream_t st, bool, double* result, std::
   }
   return CCS_OK;
 }
+
+// ------------------------------------------------------------ ensemble plan
+struct EnsPlan {
+  bool fp64 = false;
+  int tile_y = 16, persistent = 0, grid = 0, ntiles = 0, nm = 0;
+  void* dmem = nullptr;     // member table, then the tile -> member map
+  void* d_desc = nullptr;
+  int* d_tiles = nullptr;
+};
+
+namespace {
+
+template <typename T, int TY, bool PERSIST>
+void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel) {
+  const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
+  const dim3 grid(static_cast<unsigned>(e->grid)), block(TX * TY);
+  const int nt = e->ntiles;
+  switch (kind) {
+    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
+    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
+    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
+  }
+}
+
+template <typename T, int TY>
+void ens_launch_ty(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel) {
+  if (e->persistent) ens_launch_k<T, TY, true>(e, s, kind, sel);
+  else ens_launch_k<T, TY, false>(e, s, kind, sel);
+}
+
+template <typename T>
+void ens_launch(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel) {
+  if (e->tile_y == 8) ens_launch_ty<T, 8>(e, s, kind, sel);
+  else if (e->tile_y == 32) ens_launch_ty<T, 32>(e, s, kind, sel);
+  else ens_launch_ty<T, 16>(e, s, kind, sel);
+}
+
+template <typename T, int TY>
+int ens_occupancy() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, stage_ens<T, kRK3s2, TY, true>, TX * TY, 0) != hipSuccess)
+    return 0;
+  return blocks;
+}
+
+template <typename T>
+int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, std::string* err) {
+  const int TY = e->tile_y;
+  std::vector<EnsMember<T>> md(nm);
+  std::vector<int> tiles;
+  for (int m = 0; m < nm; ++m) {
+    const SplitState* s = mem[m];
+    EnsMember<T>& d = md[m];
+    d.a = base_args<T>(s);
+    d.a.nbx = (s->n + TX - 1) / TX;
+    d.a.nby = (s->n + TY - 1) / TY;
+    d.st = statics_of<T>(s);
+    for (int b = 0; b < 3; ++b)
+      for (int k = 0; k < 3; ++k) d.buf[b][k] = static_cast<T*>(s->buf[b][k]);
+    for (int k = 0; k < 9; ++k) d.tend[k] = static_cast<T*>(s->tend[k]);
+    d.inc_half = static_cast<T>(0.5 * s->cfg.dt / s->cfg.radius);
+    d.inc_full = static_cast<T>(s->cfg.dt / s->cfg.radius);
+    d.tile0 = static_cast<int>(tiles.size());
+    for (int q = 0; q < 6; ++q)
+      for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 11; ++k) d.hs[q][r][k] = s->hs.r[q][r][k];
+    tiles.insert(tiles.end(), static_cast<std::size_t>(d.a.nbx) * d.a.nby * 6, m);
+  }
+  e->ntiles = static_cast<int>(tiles.size());
+  const std::size_t bd = (md.size() * sizeof(EnsMember<T>) + 255) / 256 * 256;
+  if (hipMalloc(&e->dmem, bd + tiles.size() * sizeof(int)) != hipSuccess) {
+    *err = "ensemble: device allocation failed";
+    return CCS_ERR_NOMEM;
+  }
+  e->d_desc = e->dmem;
+  e->d_tiles = reinterpret_cast<int*>(static_cast<char*>(e->dmem) + bd);
+  if (hipMemcpy(e->d_desc, md.data(), md.size() * sizeof(EnsMember<T>), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    *err = "ensemble: table upload failed";
+    return CCS_ERR_DEVICE;
+  }
+  if (e->persistent) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) {
+      *err = "ensemble: cannot query the CU count";
+      return CCS_ERR_DEVICE;
+    }
+    const int occ = (TY == 8) ? ens_occupancy<T, 8>() : (TY == 32) ? ens_occupancy<T, 32>() : ens_occupancy<T, 16>();
+    e->grid = std::min(e->ntiles, cus * std::max(occ, 1));
+  } else {
+    e->grid = e->ntiles;
+  }
+  return CCS_OK;
+}
+
+template <typename T>
+void ens_step_t(const EnsPlan* e, hipStream_t st, int scheme) {
+  if (scheme == CCS_RK3_TVD) {
+    ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0});
+    ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0});
+    ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0});
+  } else {
+    ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0});
+    ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0});
+    ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1});
+    ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0});
+  }
+}
+
+}  // namespace
+
+int ens_plan_create(SplitState* const* mem, int nm, int tile_y, int persistent, int device, EnsPlan** out,
+                    std::string* err) {
+  if (nm <= 0 || !mem || (tile_y != 8 && tile_y != 16 && tile_y != 32)) {
+    *err = "ensemble: invalid plan (members, tile height 8/16/32)";
+    return CCS_ERR_INVALID;
+  }
+  const bool fp64 = mem[0]->fp64;
+  for (int m = 0; m < nm; ++m)
+    if (!mem[m] || mem[m]->fp64 != fp64 || !mem[m]->mesh_set) {
+      *err = "ensemble: every member needs the same precision and a mesh";
+      return CCS_ERR_ORDER;
+    }
+  EnsPlan* e = new EnsPlan();
+  e->fp64 = fp64;
+  e->tile_y = tile_y;
+  e->persistent = persistent ? 1 : 0;
+  e->nm = nm;
+  const int rc = fp64 ? ens_build<double>(e, mem, nm, device, err) : ens_build<float>(e, mem, nm, device, err);
+  if (rc != CCS_OK) {
+    ens_plan_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return CCS_OK;
+}
+
+void ens_plan_destroy(EnsPlan* e) {
+  if (!e) return;
+  if (e->dmem) (void)hipFree(e->dmem);
+  delete e;
+}
+
+void ens_enqueue_step(const EnsPlan* e, hipStream_t st, int scheme) {
+  if (e->fp64) ens_step_t<double>(e, st, scheme);
+  else ens_step_t<float>(e, st, scheme);
+}
+
+void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st) {
+  if (e->fp64) ens_launch<double>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0});
+  else ens_launch<float>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0});
+}
+
+int ens_plan_grid(const EnsPlan* e) { return e->grid; }
+int ens_plan_tile_y(const EnsPlan* e) { return e->tile_y; }
+int ens_plan_persistent(const EnsPlan* e) { return e->persistent; }
+int ens_plan_tiles(const EnsPlan* e) { return e->ntiles; }
 
 }  // namespace ccs

@@ -191,6 +191,57 @@ void ccs_destroy(ccs_handle* h);
 const char* ccs_last_error(const ccs_handle* h);
 const char* ccs_version(void);
 
+/* ------------------------------------------------------- batched ensemble */
+/* Many independent runs, each with its own resolution, mesh, initial state
+ * and time step, advanced together: one kernel launch per RK stage covers the
+ * tiles of every member.  There is no reference counterpart (the reference
+ * runs one member per process, ccsm.F90:2-73); each member is exactly the
+ * run ccs_create/ccs_step would make with the same inputs.  Split-precision
+ * paths only (CCS_FP32_SPLIT, CCS_FP64_FAST). */
+typedef struct ccs_ensemble ccs_ensemble;
+
+typedef struct ccs_ensemble_config {
+  int n_members;
+  int halo;              /* >= 3, shared by all members                      */
+  int precision;         /* CCS_FP32_SPLIT or CCS_FP64_FAST                   */
+  int scheme;            /* CCS_RK3_TVD (RK4 is rejected for ensembles)      */
+  int device;
+  int use_graph;         /* replay steps from captured HIP graphs             */
+  int tile_y;            /* 8, 16, 32; 0 = chosen by the autotuner            */
+  int persistent;        /* 0 one workgroup per tile, 1 resident grid that
+                            loops over tiles, -1 = chosen by the autotuner    */
+  double radius, gravity, omega;
+} ccs_ensemble_config;
+
+/* n, dt, dx: per member (cells per panel edge, seconds, radians). */
+int ccs_ensemble_create(const ccs_ensemble_config* cfg, const int* n,
+                        const double* dt, const double* dx, ccs_ensemble** out);
+/* As ccs_set_mesh / ccs_set_state / ccs_get_state / ccs_diagnostics, for one
+ * member (0-based).  Arrays have that member's padded size. */
+int ccs_ensemble_set_mesh(ccs_ensemble* e, int member, const double* sqrtG,
+                          const double* G, const double* IG, const double* A,
+                          const double* IA, const double* f, const double* phis);
+int ccs_ensemble_set_state(ccs_ensemble* e, int member, const double* phiG,
+                           const double* u, const double* v);
+int ccs_ensemble_get_state(ccs_ensemble* e, int member, double* phiG, double* u,
+                           double* v, double* phi, double* uc, double* vc,
+                           double* zonal, double* merid);
+int ccs_ensemble_diagnostics(ccs_ensemble* e, int member, double* mass,
+                             double* energy);
+/* Advance every member nsteps of its own dt. */
+int ccs_ensemble_step(ccs_ensemble* e, int nsteps);
+/* Time tile heights 8/16/32 x {one workgroup per tile, resident grid} on the
+ * members' tendency (no state change) and keep the fastest where the config
+ * left the choice open; reports the choice and its mean stage time.  Runs by
+ * itself on the first ccs_ensemble_step if not called. */
+int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persistent,
+                          double* stage_ms);
+/* Wall time of nsteps steps measured with HIP events on the ensemble stream. */
+int ccs_ensemble_time_steps(ccs_ensemble* e, int nsteps, double* total_ms);
+int ccs_ensemble_synchronize(ccs_ensemble* e);
+void ccs_ensemble_destroy(ccs_ensemble* e);
+const char* ccs_ensemble_last_error(const ccs_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif
