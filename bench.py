@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-members", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=60)
     ap.add_argument("--no-fp64", action="store_true")
     return ap.parse_args()
 
@@ -96,11 +99,85 @@ def cpu_baseline(ccs, mesh, s0, dt, budget_s):
                       f"{el:.1f} s on {cores} host threads"}
 
 
+def ensemble_bench(ccs, members, steps, warmup):
+    """64 independent runs, N drawn from {48, 96, 144, 192}, per-member seeds
+    for bumps and mountain, CFL 0.5 steps of their own: batched (one launch
+    per RK stage for all members, autotuned) against the same runs made one
+    after another; per-member error against the FP64 parity path."""
+    import numpy as np
+
+    from conformal_cubed_sphere_amd.ensemble_run import build_members
+
+    runs, sizes = build_members(members)
+    cells = sum(6 * n * n for n in sizes)
+
+    ens = ccs.Ensemble([r[0] for r in runs], [r[2] for r in runs], precision="fp32")
+    for k, r in enumerate(runs):
+        ens.set_state(k, r[1])
+    choice = ens.autotune()
+    ens.step(warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.synchronize()
+    wall_b = time.perf_counter() - t0
+
+    cores = []
+    for r in runs:
+        c = ccs.DynamicalCore(r[0], r[2], precision="fp32")
+        c.set_state(r[1])
+        c.step(warmup)
+        c.synchronize()
+        cores.append(c)
+    t0 = time.perf_counter()
+    for c in cores:  # one run after another
+        c.step(steps)
+        c.synchronize()
+    wall_s = time.perf_counter() - t0
+    for c in cores:
+        c.close()
+
+    # error against FP64, beside the method's own floor: an FP64 run from the
+    # initial state perturbed at FP32 rounding level (2^-24 relative)
+    l2, floor, drift = [], [], []
+    prng = np.random.default_rng(2024)
+    for k, r in enumerate(runs):
+        got = ens.get_state(k)
+        sp = r[1].copy()
+        for f in ("phiG", "u", "v"):
+            a = getattr(sp, f)
+            a *= 1.0 + 2.0 ** -24 * prng.standard_normal(a.shape)
+        fin = []
+        for st0, comp in ((r[1], True), (sp, False)):
+            with ccs.DynamicalCore(r[0], r[2], precision="fp64") as c:
+                c.set_state(st0, with_companions=comp)
+                if comp:
+                    m0, _ = c.diagnostics()
+                c.step(warmup + steps)
+                fin.append(c.get_state())
+        mesh = r[0]
+        area = mesh.interior(mesh.areaCell)
+        href = mesh.interior(fin[0].phi) / GRAVITY
+        l2.append(ccs.error_norms(mesh.interior(got.phi) / GRAVITY, href, area)[1])
+        floor.append(ccs.error_norms(mesh.interior(fin[1].phi) / GRAVITY, href, area)[1])
+        drift.append(abs(ens.diagnostics(k)[0] - m0) / m0)
+    ens.close()
+    counts = {str(n): sizes.count(n) for n in sorted(set(sizes))}
+    return {"members": members, "members_per_n": counts, "interior_cells": cells, "steps": steps,
+            "value_batched": cells * steps / wall_b, "value_serial": cells * steps / wall_s,
+            "unit": "member cell-updates/s", "speedup_batched_over_serial": wall_s / wall_b,
+            "autotune": choice, "precision": "fp32",
+            "l2_h_vs_fp64_median": float(np.median(l2)), "l2_h_vs_fp64_max": float(np.max(l2)),
+            "l2_h_fp64_perturbed_median": float(np.median(floor)), "l2_h_fp64_perturbed_max": float(np.max(floor)),
+            "mass_drift_max": float(np.max(drift)),
+            "note": f"error after {warmup + steps} steps of each member's own dt"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
-              "the multi-GPU panel sharding is a design in DESIGN.md, not a runnable path", file=sys.stderr)
+              "the multi-GPU panel sharding is a design only, not a runnable path", file=sys.stderr)
         sys.exit(2)
     import conformal_cubed_sphere_amd as ccs
 
@@ -183,6 +260,8 @@ def main():
         out["l2_h_compared_after_hours"] = nadv * dt / 3600.0
     else:
         core.close()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(ccs, args.ens_members, args.ens_steps, 5)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ccs, mesh, s0, dt, args.cpu_seconds)
     print(json.dumps(out), flush=True)

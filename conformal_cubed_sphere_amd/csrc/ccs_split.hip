@@ -603,13 +603,18 @@ __global__ void __launch_bounds__(kDT) split_diag(Prog<const T> q, const double*
   }
 }
 
-__global__ void split_diag_final(const double* partials, int nb, double* result) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// nb <= 4 * kDT partial blocks; all threads stage them in LDS (one round of
+// loads instead of nb dependent ones), thread 0 sums them in block order.
+__global__ void __launch_bounds__(kDT) split_diag_final(const double* partials, int nb, double* result) {
+  __shared__ double sp[4 * 4 * kDT];
+  for (int k = threadIdx.x; k < 4 * nb; k += kDT) sp[k] = partials[k];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   KSum M{0., 0.}, E{0., 0.};
   double cm = 0., ce = 0.;
   for (int b = 0; b < nb; ++b) {
-    M.add(partials[4 * b + 0]); cm += partials[4 * b + 1];
-    E.add(partials[4 * b + 2]); ce += partials[4 * b + 3];
+    M.add(sp[4 * b + 0]); cm += sp[4 * b + 1];
+    E.add(sp[4 * b + 2]); ce += sp[4 * b + 3];
   }
   result[0] = M.s + (M.c + cm);
   result[1] = E.s + (E.c + ce);
@@ -979,7 +984,8 @@ int split_diagnostics(SplitState* s, hipStream_t st, bool, double* result, std::
     hipLaunchKernelGGL(split_diag<float>, dim3(kSplitDiagBlocks), dim3(kDT), 0, st, cprog_of<float>(s, 0),
                        s->pref64_d, s->sqrtG_d, s->phis_d, s->n, s->h, s->cfg.radius, s->partials);
   }
-  hipLaunchKernelGGL(split_diag_final, dim3(1), dim3(64), 0, st, s->partials, kSplitDiagBlocks, s->result);
+  static_assert(kSplitDiagBlocks <= 4 * kDT, "partials fit the final block's LDS");
+  hipLaunchKernelGGL(split_diag_final, dim3(1), dim3(kDT), 0, st, s->partials, kSplitDiagBlocks, s->result);
   if (hipMemcpyAsync(result, s->result, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess) {
     *err = "split_diagnostics: kernel or copy failed";
