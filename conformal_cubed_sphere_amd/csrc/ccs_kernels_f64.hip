@@ -88,8 +88,16 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
   __shared__ double sFyF[(TY + 1) * TX], sEyF[(TY + 1) * TX];
 
   const int n = a.n, h = a.h, P = n + 2 * h;
-  const int p = blockIdx.z;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD gets a
+  // contiguous run of tiles (x fastest, then y, then panel) so the borders
+  // shared by neighbouring tiles are fetched into one L2.
+  const unsigned nbx = (static_cast<unsigned>(a.n) + TX - 1) / TX, nby = (static_cast<unsigned>(a.n) + TY - 1) / TY;
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned xcd = b & 7u;
+  const unsigned t = xcd * (nb >> 3) + min(xcd, nb & 7u) + (b >> 3);
+  const unsigned tq = t / nbx;
+  const int p = static_cast<int>(tq / nby);
+  const int i0 = static_cast<int>(t - tq * nbx) * TX, j0 = static_cast<int>(tq - static_cast<unsigned>(p) * nby) * TY;
   const int tid = threadIdx.x;
 
   // ---- phase 1: pointwise terms on the cross-shaped extended tile
@@ -386,7 +394,7 @@ __global__ void __launch_bounds__(kDiagThreads) diag_final_f64(const double* par
 
 void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a, const StaticsF64& st,
                       const HaloScatter& hs) {
-  const dim3 grid((a.n + TX - 1) / TX, (a.n + TY - 1) / TY, 6);
+  const dim3 grid(static_cast<unsigned>(((a.n + TX - 1) / TX) * ((a.n + TY - 1) / TY) * 6));
   const dim3 block(NT);
 #define CCS_LAUNCH(K)                                                                   \
   do {                                                                                 \

@@ -28,12 +28,17 @@
 //            are computed once and feed both reconstructions the window
 //            gives (the value at its centre cell's right face and at its left
 //            face; the reference recomputes them per face,
-//            spatial_operators_mod.F90:80-99).  The Rusanov flux of a face,
-//            F = (F_L + F_R)/2 - lambda (P_R - P_L)/2, is split into the half
-//            each side contributes, (F_L + lambda P_L)/2 and
-//            (F_R - lambda P_R)/2, so every window writes its own share and
-//            no separate flux pass (or barrier) is needed; lambda is computed
-//            from the cell values beside the face by each contributor.
+//            spatial_operators_mod.F90:80-99).  Windows are evaluated on
+//            differences from their centre value, so reconstructions come
+//            out as small deviations; the Rusanov flux of a face,
+//            F = (F_L + F_R)/2 - lambda (P_R - P_L)/2, splits into a centre
+//            term (F_A + F_B)/2 - lambda (P_B - P_A)/2 and the share each side's
+//            deviations contribute.  Every window writes its own share, no
+//            separate flux pass (or barrier) is needed, and no sum or
+//            difference of large values enters the divergence: in FP32 the
+//            flux differences keep their relative precision (in the absolute
+//            form they lose 3-5 digits to cancellation and the round-off
+//            drives long runs unstable).
 //   phase 3  divergence, vorticity, RK update, halo scatter with the signed
 //            permutation of the panel-basis change (ghost_mod.f90:100-153).
 #include <hip/hip_runtime.h>
@@ -104,29 +109,36 @@ struct SplitArgs {
 // indicators reverse order, which swaps tau40 and tau41.
 //
 // With a_k, b_k the second and first differences of the Jiang-Shu
-// indicators, every candidate stencil is q2 + a_k/12 +- b_k/4:
+// indicators, every candidate stencil is q2 + a_k/12 +- b_k/4 (q2 dropped:
+// values are deviations from the window centre):
 //   right face: s1 = q2 + a1/12 + b1/4, s2 = q2 + a2/12 - b2/4, s3 = q2 + a3/12 - b3/4
 //   left face : s1 = q2 + a3/12 + b3/4, s2 = q2 + a2/12 + b2/4, s3 = q2 + a1/12 - b1/4
 // and the branch weights (1/3,2/3,0), (0,2/3,1/3) are the unnormalised
 // weights (1,2,0), (0,2,1) fed through the same normalisation.
 template <typename T>
 struct WenoWin {
-  T c1, c2, c3;     // q2 + a_k/12
+  T c1, c2, c3;     // a_k/12 (relative to q2)
   T h1, h2, h3;     // b_k/2
   T t40, t41, mb;   // indicator differences and min
   T tr1, tr2, tr3;  // 1 + tau5 / (eps + beta_k)
 };
 
+// The window is evaluated relative to its centre value q2: the inputs are
+// the differences q_k - q2 (exact in FP32 for neighbouring values of one
+// sign, by Sterbenz' lemma) and the reconstructions are returned as
+// deviations from q2.  The weights only see differences, so they are those
+// of the absolute form; the absolute face value is q2 + deviation.
 template <typename T>
 __device__ __forceinline__ WenoWin<T> weno_window(T q0, T q1, T q2, T q3, T q4, T eps) {
 #pragma clang fp contract(fast)
   WenoWin<T> w;
-  const T a1 = T(-2) * q1 + (q0 + q2);
-  const T a2 = T(-2) * q2 + (q1 + q3);
-  const T a3 = T(-2) * q3 + (q2 + q4);
-  w.h1 = T(0.5) * a1 + (q2 - q1);
-  w.h2 = T(0.5) * (q1 - q3);
-  w.h3 = T(0.5) * a3 + (q2 - q3);
+  const T d0 = q0 - q2, d1 = q1 - q2, d3 = q3 - q2, d4 = q4 - q2;
+  const T a1 = T(-2) * d1 + d0;
+  const T a2 = d1 + d3;
+  const T a3 = T(-2) * d3 + d4;
+  w.h1 = T(0.5) * a1 - d1;
+  w.h2 = T(0.5) * (d1 - d3);
+  w.h3 = T(0.5) * a3 - d3;
   const T k13 = T(13) / T(12);
   const T be1 = (k13 * a1) * a1 + w.h1 * w.h1;
   const T be2 = (k13 * a2) * a2 + w.h2 * w.h2;
@@ -139,9 +151,9 @@ __device__ __forceinline__ WenoWin<T> weno_window(T q0, T q1, T q2, T q3, T q4, 
   w.tr2 = t5 * rcp(eps + be2) + T(1);
   w.tr3 = t5 * rcp(eps + be3) + T(1);
   const T k12 = T(1) / T(12);
-  w.c1 = a1 * k12 + q2;
-  w.c2 = a2 * k12 + q2;
-  w.c3 = a3 * k12 + q2;
+  w.c1 = a1 * k12;
+  w.c2 = a2 * k12;
+  w.c3 = a3 * k12;
   return w;
 }
 
@@ -311,10 +323,17 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   };
 
   // ---- phase 2: WENO windows and face shares.  Thread (tx, ty) owns the
-  // windows centred on its own cell along x and along y; its own shares stay
-  // in registers for phase 3.
+  // windows centred on its own cell along x and along y.  With face values
+  // written as centre value + deviation, the flux of the face between cells
+  // A (left/below) and B is
+  //   F(f) = C(f) + (dF_A + lam dP_A)/2 + (dF_B - lam dP_B)/2,
+  //   C(f) = (F_A + F_B)/2 - lam (P_B - P_A)/2,
+  // dF_A, dP_A the deviations of A's window at f, dF_B, dP_B those of B's.
+  // Each window writes its own share (small numbers); phase 3 adds the
+  // centre terms from differences of neighbouring centre values.  Own shares
+  // and Rusanov speeds stay in registers.
   const int tx = tid % TX, ty = tid / TX;
-  T fr, fl, er, el, fu, fd, eu, ed;
+  T fr, fl, er, el, fu, fd, eu, ed, lr, ll, lu, ld;
   {
     const int l = (ty + 3) * EX + tx + 3;
     const int fx = ty * (TX + 1) + tx;   // left x-face of the cell
@@ -323,7 +342,8 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       const WenoWin<T> wp = weno_window(sPt[l - 2], sPt[l - 1], sPt[l], sPt[l + 1], sPt[l + 2], a.epsP);
       const WenoWin<T> wf = weno_window(sFx[l - 2], sFx[l - 1], sFx[l], sFx[l + 1], sFx[l + 2], a.epsF);
       const WenoWin<T> we = weno_window(sE[l - 2], sE[l - 1], sE[l], sE[l + 1], sE[l + 2], a.epsE);
-      const T lr = lam_x(l), ll = lam_x(l - 1);
+      lr = lam_x(l);
+      ll = lam_x(l - 1);
       fr = T(0.5) * (weno_right(wf) + lr * weno_right(wp));
       fl = T(0.5) * (weno_left(wf) - ll * weno_left(wp));
       er = T(0.5) * weno_right(we);
@@ -337,7 +357,8 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       const WenoWin<T> wp = weno_window(sPt[l - 2 * EX], sPt[l - EX], sPt[l], sPt[l + EX], sPt[l + 2 * EX], a.epsP);
       const WenoWin<T> wf = weno_window(sFy[l - 2 * EX], sFy[l - EX], sFy[l], sFy[l + EX], sFy[l + 2 * EX], a.epsF);
       const WenoWin<T> we = weno_window(sE[l - 2 * EX], sE[l - EX], sE[l], sE[l + EX], sE[l + 2 * EX], a.epsE);
-      const T lu = lam_y(l), ld = lam_y(l - EX);
+      lu = lam_y(l);
+      ld = lam_y(l - EX);
       fu = T(0.5) * (weno_right(wf) + lu * weno_right(wp));
       fd = T(0.5) * (weno_left(wf) - ld * weno_left(wp));
       eu = T(0.5) * weno_right(we);
@@ -401,12 +422,17 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   const T rdx = a.rdx;
   const int lc = (ty + 3) * EX + (tx + 3);
   const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
-  // face flux = the two shares; both cells beside a face form the same sum
-  // (IEEE addition commutes), so the fluxes telescope and mass is conserved
-  const T dFx = ((fr + xmF[fx + 1]) - (xpF[fx] + fl)) * rdx;
-  const T dEx = ((er + xmE[fx + 1]) - (xpE[fx] + el)) * rdx;
-  const T dFy = ((fu + ymF[fy + TX]) - (ypF[fy] + fd)) * rdx;
-  const T dEy = ((eu + ymE[fy + TX]) - (ypE[fy] + ed)) * rdx;
+  // F(right face) - F(left face): centre terms from differences of
+  // neighbouring centre values, plus the face shares.  Both cells beside a
+  // face form the same share sum and the same lam (P_B - P_A) product
+  // (identical operands, IEEE addition commutes), so the fluxes telescope.
+  const T pc = sPt[lc];
+  const T cx = T(0.5) * (sFx[lc + 1] - sFx[lc - 1]) - T(0.5) * (lr * (sPt[lc + 1] - pc) - ll * (pc - sPt[lc - 1]));
+  const T cy = T(0.5) * (sFy[lc + EX] - sFy[lc - EX]) - T(0.5) * (lu * (sPt[lc + EX] - pc) - ld * (pc - sPt[lc - EX]));
+  const T dFx = (cx + ((fr + xmF[fx + 1]) - (xpF[fx] + fl))) * rdx;
+  const T dEx = (T(0.5) * (sE[lc + 1] - sE[lc - 1]) + ((er + xmE[fx + 1]) - (xpE[fx] + el))) * rdx;
+  const T dFy = (cy + ((fu + ymF[fy + TX]) - (ypF[fy] + fd))) * rdx;
+  const T dEy = (T(0.5) * (sE[lc + EX] - sE[lc - EX]) + ((eu + ymE[fy + TX]) - (ypE[fy] + ed))) * rdx;
   const T dvdx = d6f(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], rdx);
   const T dudy = d6f(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX], sU[lc + 3 * EX], rdx);
   const unsigned go = (gt + __umul24(static_cast<unsigned>(ty + 3), P) + static_cast<unsigned>(tx + 3)) * W;

@@ -130,3 +130,27 @@ def test_one_day_c32_normalised_l2(ccs, orc):
     l1, l2, linf = ccs.error_norms(h_gpu, h_orc, area)
     print(f"  normalised L1/L2/Linf(h) = {l1:.3e} {l2:.3e} {linf:.3e}")
     assert l2 <= 1e-12
+
+
+@pytest.mark.slow
+def test_one_day_c180_matches_cpu_oracle(ccs, orc):
+    """The drop-in FP64 path at C180 (the benchmark recipe: a = 6.371e6 m,
+    seeded jet + 8 bumps, flat bottom, SSP-RK3 at CFL 0.5) against the CPU
+    oracle after one simulated day: normalised L2(h) <= 1e-12 (in practice
+    the two are equal bit for bit)."""
+    mesh = ccs.Mesh(180, halo=3, radius=6.371e6, omega=7.292e-5)
+    s0 = ccs.init_case(mesh, case_num=7, seed=1, n_bumps=8, mountain=False)
+    dt, nsteps = ccs.cfl_time_step(mesh, s0, 0.5, 86400.0)
+    print(f"1 day C180: {nsteps} steps of dt={dt:.3f}", flush=True)
+    with ccs.DynamicalCore(mesh, dt) as core:
+        core.set_state(s0)
+        core.step(nsteps)
+        s = core.get_state()
+    o = orc.Oracle(mesh, s0, dt)
+    o.steps(nsteps)
+    area = mesh.interior(mesh.areaCell)
+    l1, l2, linf = ccs.error_norms(mesh.interior(s.phi) / 9.80616, mesh.interior(o.a["phi"]) / 9.80616, area)
+    print(f"  normalised L1/L2/Linf(h) = {l1:.3e} {l2:.3e} {linf:.3e}", flush=True)
+    assert l2 <= 1e-12
+    mask = ~mesh.corner_mask().ravel()
+    print(f"  bitwise equal: {np.array_equal(s.phiG[mask], o.a['phiG'][mask])}", flush=True)
