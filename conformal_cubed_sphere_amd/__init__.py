@@ -12,11 +12,15 @@ from .core import (  # noqa: F401
     CcsError,
     DynamicalCore,
     Ensemble,
+    History,
     Mesh,
     State,
     cfl_time_step,
     error_norms,
     init_case,
+    read_mesh_file,
+    write_mesh_file,
 )
 
-__all__ = ["CcsError", "DynamicalCore", "Ensemble", "Mesh", "State", "cfl_time_step", "error_norms", "init_case"]
+__all__ = ["CcsError", "DynamicalCore", "Ensemble", "History", "Mesh", "State", "cfl_time_step", "error_norms", "init_case", "read_mesh_file",
+           "write_mesh_file"]

@@ -74,6 +74,11 @@ class CcsEnsembleConfig(ctypes.Structure):
     ]
 
 
+class CcsHistoryDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("halo", ctypes.c_int), ("case_num", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("dx_deg", ctypes.c_double), ("dt", ctypes.c_double)]
+
+
 _I = ctypes.POINTER(ctypes.c_int)
 
 # Every entry point declared in include/ccs.h, with its ctypes signature.
@@ -95,6 +100,12 @@ SIGNATURES = {
     "ccs_destroy": (None, [ctypes.c_void_p]),
     "ccs_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "ccs_version": (ctypes.c_char_p, []),
+    "ccs_mesh_file_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "ccs_mesh_file_read": (ctypes.c_int, [ctypes.c_char_p, _I, _D, _D, _D]),
+    "ccs_history_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CcsHistoryDesc), _D, _D, _D, _D,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "ccs_history_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D]),
+    "ccs_history_close": (ctypes.c_int, [ctypes.c_void_p]),
     "ccs_ensemble_create": (ctypes.c_int, [ctypes.POINTER(CcsEnsembleConfig), _I, _D, _D,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "ccs_ensemble_set_mesh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D]),

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -347,3 +348,60 @@ class Ensemble:
         self._chk("ccs_ensemble_diagnostics", lib.ccs_ensemble_diagnostics(self._h, member, ctypes.byref(mass),
                                                                            ctypes.byref(energy)))
         return mass.value, energy.value
+
+
+def write_mesh_file(path: str, n: int):
+    """Mesh file in the reference generator's schema (output_mod.f90:25-83),
+    readable by the dycore's read_mesh (io_mod.F90:37-117)."""
+    _check("ccs_mesh_file_write", lib.ccs_mesh_file_write(os.fsencode(path), int(n)))
+
+
+def read_mesh_file(path: str):
+    """(n, lon_deg, lat_deg, jab) of the cell centres of a mesh file
+    (classic or 64-bit-offset netCDF), laid out as ccs_mesh_geometry's."""
+    n = ctypes.c_int()
+    _check("ccs_mesh_file_read", lib.ccs_mesh_file_read(os.fsencode(path), ctypes.byref(n), None, None, None))
+    ni = 6 * n.value * n.value
+    lon, lat, jab = np.zeros(ni), np.zeros(ni), np.zeros(6 * ni)
+    _check("ccs_mesh_file_read", lib.ccs_mesh_file_read(os.fsencode(path), ctypes.byref(n), dptr(lon), dptr(lat),
+                                                        dptr(jab)))
+    return n.value, lon, lat, jab
+
+
+class History:
+    """History file ccsm_output.nc (io_mod.F90:309-541): history_init at
+    construction, history_write_stat per output slot (1-based)."""
+
+    def __init__(self, path: str, mesh: Mesh, phis, dt: float, case_num: int):
+        d = _lib.CcsHistoryDesc(n=mesh.n, halo=mesh.halo, case_num=int(case_num), pad_=0,
+                                dx_deg=90.0 / mesh.n, dt=float(dt))
+        self._phis = np.ascontiguousarray(phis, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check("ccs_history_create", lib.ccs_history_create(os.fsencode(path), ctypes.byref(d), dptr(mesh.lon),
+                                                            dptr(mesh.lat), dptr(mesh.areaCell), dptr(self._phis),
+                                                            ctypes.byref(h)))
+        self._h = h
+        self.mesh = mesh
+
+    def write(self, slot: int, s: State):
+        arrs = [np.ascontiguousarray(getattr(s, k), dtype=np.float64) for k in
+                ("phi", "u", "v", "uc", "vc", "zonal", "merid")]
+        _check("ccs_history_write", lib.ccs_history_write(self._h, int(slot), dptr(arrs[0]), dptr(self._phis),
+                                                          *(dptr(a) for a in arrs[1:])))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check("ccs_history_close", lib.ccs_history_close(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -207,7 +207,7 @@ cd w_of_z(cd zt, cd guess, int* iters) {
       const cd wn = w - dw;
       const double ang = std::arg(wn);
       const double er = (wn.real() + 1.0) * (wn.real() + 1.0) + wn.imag() * wn.imag();
-      if (ang >= -1e-9 && ang <= M_PI / 4 + 1e-9 && er < 2.0) break;
+      if (ang >= -1e-9 && ang <= M_PI / 4 + 1e-9 && er <= 2.0 + 1e-12) break;
       dw *= 0.5;
     }
     w -= dw;
@@ -232,50 +232,89 @@ const double kRot[6][3][3] = {
 
 }  // namespace
 
-// Solve w for the cells of one face that lie in the fundamental triangle and
-// return w and dw/dz for every cell of the face (n*n, i fastest).
-static void face_map(int n, std::vector<cd>& w, std::vector<cd>& dwdz) {
-  w.assign(static_cast<std::size_t>(n) * n, cd(0, 0));
+// w and dw/dz at the points (coord[i], coord[j]) of one face (m*m, i
+// fastest), coord ascending and symmetric about 0 (it may contain 0 and the
+// edges +-1).  Newton solves cover the fundamental triangle
+// 0 <= y <= x (continuation along rows from the diagonal); the rest follows
+// from the face's symmetries.  The cube vertex, where dz/dw is singular, is
+// set from its closed form w = (1 + i)/(sqrt(3) + 1).
+void map_points(const std::vector<double>& coord, std::vector<cd>& w, std::vector<cd>& dwdz) {
+  const int m = static_cast<int>(coord.size());
+  w.assign(static_cast<std::size_t>(m) * m, cd(0, 0));
   dwdz.assign(w.size(), cd(0, 0));
   const double k = kappa();
-  auto coord = [n](int i) { return -1.0 + (2.0 * i + 1.0) / n; };  // 0-based
-  // Fundamental triangle: 0-based i, j >= n/2 and i >= j.
-  const int h0 = n / 2;
-  std::vector<cd> tri(static_cast<std::size_t>(n) * n, cd(0, 0));
-  cd diag_guess = k * cd(coord(h0), coord(h0));
-  for (int j = h0; j < n; ++j) {
+  const int h0 = m / 2;  // first index with coord >= 0
+  const cd vertex = cd(1.0, 1.0) / (std::sqrt(3.0) + 1.0);
+  std::vector<cd> tri(static_cast<std::size_t>(m) * m, cd(0, 0));
+  cd diag_guess = k * cd(coord[h0], coord[h0]);
+  for (int j = h0; j < m; ++j) {
     cd guess = diag_guess;
-    for (int i = j; i < n; ++i) {
-      const cd zt(coord(i), coord(j));
-      const cd wi = w_of_z(zt, guess, nullptr);
-      tri[static_cast<std::size_t>(j) * n + i] = wi;
+    for (int i = j; i < m; ++i) {
+      const cd zt(coord[i], coord[j]);
+      cd wi = (coord[i] == 1.0 && coord[j] == 1.0) ? vertex : w_of_z(zt, guess, nullptr);
+      if (coord[i] == 1.0 && coord[j] != 1.0) {
+        // panel edge x = 1: w lies on the stereographic image of the edge
+        // great circle, |w + 1| = sqrt(2); project the Newton result onto it
+        // so that both panels sharing the edge map it to the same points
+        const cd c = wi + 1.0;
+        wi = std::sqrt(2.0) * c / std::abs(c) - 1.0;
+      }
+      tri[static_cast<std::size_t>(j) * m + i] = wi;
       if (i == j) {
         // extrapolate along the diagonal for the next row's start
         diag_guess = wi;
         if (j > h0) {
-          const cd prev = tri[static_cast<std::size_t>(j - 1) * n + (j - 1)];
+          const cd prev = tri[static_cast<std::size_t>(j - 1) * m + (j - 1)];
           diag_guess = wi + 0.5 * (wi - prev);
         }
       }
       guess = wi;
     }
   }
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) {
-      const double x = coord(i), y = coord(j);
-      const int ia = (x >= 0) ? i : n - 1 - i;  // |x| index
-      const int ja = (y >= 0) ? j : n - 1 - j;  // |y| index
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      const double x = coord[i], y = coord[j];
+      const int ia = (x >= 0) ? i : m - 1 - i;  // |x| index
+      const int ja = (y >= 0) ? j : m - 1 - j;  // |y| index
       cd wa;
       if (ia >= ja) {
-        wa = tri[static_cast<std::size_t>(ja) * n + ia];
+        wa = tri[static_cast<std::size_t>(ja) * m + ia];
       } else {
-        const cd wt = tri[static_cast<std::size_t>(ia) * n + ja];
+        const cd wt = tri[static_cast<std::size_t>(ia) * m + ja];
         wa = cd(0, 1) * std::conj(wt);  // reflection across the diagonal
       }
       const cd wij((x >= 0 ? 1.0 : -1.0) * wa.real(), (y >= 0 ? 1.0 : -1.0) * wa.imag());
-      w[static_cast<std::size_t>(j) * n + i] = wij;
-      dwdz[static_cast<std::size_t>(j) * n + i] = k * std::pow(face_form(wij), 0.25);
+      w[static_cast<std::size_t>(j) * m + i] = wij;
+      dwdz[static_cast<std::size_t>(j) * m + i] = k * std::pow(face_form(wij), 0.25);
     }
+}
+
+// Position on the unit sphere and Jacobian d(xyz)/d(map angle) (3x2,
+// column-major, per radian) of the point w of panel p (0-based).
+void panel_point(int p, cd w, cd dwdz, double g[3], double jab[6]) {
+  const double(*R)[3] = kRot[p];
+  const double quarter_pi = 2.0 * std::asin(1.0) / 4.0;  // d(map angle)/d(xm)
+  const double r = w.real(), s = w.imag();
+  const double D = 1.0 + r * r + s * s;
+  const double v[3] = {2.0 * r / D, 2.0 * s / D, (1.0 - r * r - s * s) / D};
+  const double D2 = D * D;
+  const double dvr[3] = {2.0 * (1.0 - r * r + s * s) / D2, -4.0 * r * s / D2, -4.0 * r / D2};
+  const double dvs[3] = {-4.0 * r * s / D2, 2.0 * (1.0 + r * r - s * s) / D2, -4.0 * s / D2};
+  const double pr = dwdz.real(), pi_ = dwdz.imag();
+  double dx_loc[3], dy_loc[3], gx[3], gy[3];
+  for (int a = 0; a < 3; ++a) {
+    dx_loc[a] = dvr[a] * pr + dvs[a] * pi_;   // d v / d xm
+    dy_loc[a] = -dvr[a] * pi_ + dvs[a] * pr;  // d v / d ym
+  }
+  for (int a = 0; a < 3; ++a) {
+    g[a] = R[a][0] * v[0] + R[a][1] * v[1] + R[a][2] * v[2];
+    gx[a] = R[a][0] * dx_loc[0] + R[a][1] * dx_loc[1] + R[a][2] * dx_loc[2];
+    gy[a] = R[a][0] * dy_loc[0] + R[a][1] * dy_loc[1] + R[a][2] * dy_loc[2];
+  }
+  for (int a = 0; a < 3; ++a) {
+    jab[a] = gx[a] / quarter_pi;
+    jab[3 + a] = gy[a] / quarter_pi;
+  }
 }
 
 }  // namespace ccs
@@ -285,32 +324,17 @@ using namespace ccs;
 extern "C" int ccs_mesh_geometry(int n, double* lon_deg, double* lat_deg, double* xyz,
                                  double* jab) {
   if (n < 4 || (n % 2) != 0) return CCS_ERR_INVALID;
+  std::vector<double> coord(n);
+  for (int i = 0; i < n; ++i) coord[i] = -1.0 + (2.0 * i + 1.0) / n;  // cell centres
   std::vector<std::complex<double>> w, dwdz;
-  face_map(n, w, dwdz);
+  map_points(coord, w, dwdz);
   const double pi = 2.0 * std::asin(1.0);
   const double r2d = 180.0 / pi;
-  const double quarter_pi = pi / 4.0;  // d(map angle)/d(xm)
   const std::size_t ncell = static_cast<std::size_t>(n) * n;
   for (int p = 0; p < 6; ++p) {
-    const double(*R)[3] = kRot[p];
     for (std::size_t c = 0; c < ncell; ++c) {
-      const double r = w[c].real(), s = w[c].imag();
-      const double D = 1.0 + r * r + s * s;
-      const double v[3] = {2.0 * r / D, 2.0 * s / D, (1.0 - r * r - s * s) / D};
-      const double D2 = D * D;
-      const double dvr[3] = {2.0 * (1.0 - r * r + s * s) / D2, -4.0 * r * s / D2, -4.0 * r / D2};
-      const double dvs[3] = {-4.0 * r * s / D2, 2.0 * (1.0 + r * r - s * s) / D2, -4.0 * s / D2};
-      const double pr = dwdz[c].real(), pi_ = dwdz[c].imag();
-      double dx_loc[3], dy_loc[3], g[3], gx[3], gy[3];
-      for (int a = 0; a < 3; ++a) {
-        dx_loc[a] = dvr[a] * pr + dvs[a] * pi_;   // d v / d xm
-        dy_loc[a] = -dvr[a] * pi_ + dvs[a] * pr;  // d v / d ym
-      }
-      for (int a = 0; a < 3; ++a) {
-        g[a] = R[a][0] * v[0] + R[a][1] * v[1] + R[a][2] * v[2];
-        gx[a] = R[a][0] * dx_loc[0] + R[a][1] * dx_loc[1] + R[a][2] * dx_loc[2];
-        gy[a] = R[a][0] * dy_loc[0] + R[a][1] * dy_loc[1] + R[a][2] * dy_loc[2];
-      }
+      double g[3], jb[6];
+      panel_point(p, w[c], dwdz[c], g, jb);
       const std::size_t cell = c + ncell * static_cast<std::size_t>(p);
       if (xyz) {
         xyz[3 * cell + 0] = g[0];
@@ -319,12 +343,8 @@ extern "C" int ccs_mesh_geometry(int n, double* lon_deg, double* lat_deg, double
       }
       if (lon_deg) lon_deg[cell] = std::atan2(g[1], g[0]) * r2d;
       if (lat_deg) lat_deg[cell] = std::atan2(g[2], std::sqrt(g[0] * g[0] + g[1] * g[1])) * r2d;
-      if (jab) {
-        for (int a = 0; a < 3; ++a) {
-          jab[6 * cell + a] = gx[a] / quarter_pi;
-          jab[6 * cell + 3 + a] = gy[a] / quarter_pi;
-        }
-      }
+      if (jab)
+        for (int a = 0; a < 6; ++a) jab[6 * cell + a] = jb[a];
     }
   }
   return CCS_OK;

@@ -34,7 +34,46 @@ module ccs_gpu_mod
     real(c_double) :: dt, dx, radius, gravity, omega
   end type ccs_config
 
+  type, bind(C) :: ccs_history_desc
+    integer(c_int) :: n, halo, case_num, pad_
+    real(c_double) :: dx_deg, dt
+  end type ccs_history_desc
+
   interface
+    integer(c_int) function ccs_mesh_file_write(path, n) bind(C, name="ccs_mesh_file_write")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: n
+    end function
+
+    integer(c_int) function ccs_mesh_file_read(path, n, lon_deg, lat_deg, jab) bind(C, name="ccs_mesh_file_read")
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: n
+      type(c_ptr), value :: lon_deg, lat_deg, jab
+    end function
+
+    integer(c_int) function ccs_history_create(path, d, lon, lat, areaCell, phis, h) bind(C, name="ccs_history_create")
+      import :: c_int, c_char, c_double, c_ptr, ccs_history_desc
+      character(kind=c_char), intent(in) :: path(*)
+      type(ccs_history_desc), intent(in) :: d
+      real(c_double), intent(in) :: lon(*), lat(*), areaCell(*), phis(*)
+      type(c_ptr) :: h
+    end function
+
+    integer(c_int) function ccs_history_write(h, slot, phi, phis, u, v, uc, vc, zonal, merid) &
+        bind(C, name="ccs_history_write")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: slot
+      real(c_double), intent(in) :: phi(*), phis(*), u(*), v(*), uc(*), vc(*), zonal(*), merid(*)
+    end function
+
+    integer(c_int) function ccs_history_close(h) bind(C, name="ccs_history_close")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
     integer(c_int) function ccs_mesh_geometry(n, lon_deg, lat_deg, xyz, jab) bind(C, name="ccs_mesh_geometry")
       import :: c_int, c_double
       integer(c_int), value :: n

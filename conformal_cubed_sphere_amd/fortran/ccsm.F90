@@ -9,15 +9,21 @@
 ! time step itself runs on the GPU through libccs.so (ccs_gpu_mod).
 !
 ! Differences, all in the optional group &gpu_opt that follows &dynamic_opt:
-!   nx         cells per panel edge.  The reference reads the grid from a
-!              netCDF mesh file (io_mod.F90:37-117); there is no netCDF here,
-!              so the conformal mesh is generated in memory (ccs_mesh_geometry).
+!   nx         cells per panel edge.  When &domain mesh_file names an existing
+!              file of the generator's schema (classic or 64-bit-offset
+!              netCDF, e.g. written by ccs_mesh_file_write) the grid is read
+!              from it as io_mod.F90:37-135 does and nx may be left unset;
+!              otherwise the conformal mesh of nx cells is generated in memory
+!              (ccs_mesh_geometry).
 !   precision  'fp64' (bit-faithful to the reference), 'fp32', 'fp64_fast'.
 !   cfl        > 0: dt = largest step <= cfl * min(dx / max Rusanov speed)
 !              that divides the run, from the initial state; when the
 !              history interval divides the run, the step count is rounded
 !              up to a multiple of the number of outputs.
 !   seed, n_bumps, mountain, u0, h_mean: case 7, the seeded balanced jet.
+!   history_file  history output of io_mod.F90:309-541 written at the
+!              initial state and every output (default 'ccsm_output.nc', the
+!              reference's name; '' for none).
 !   radius, gravity, omega, device, use_graph, final_state_file.
 module ccs_params_mod
   use iso_c_binding
@@ -42,6 +48,8 @@ module ccs_params_mod
   real(8) :: radius = 6371229.d0, gravity = 9.80616d0, omega = 7.292d-5
   integer :: device = 0, use_graph = 1
   character(len=200) :: final_state_file = ''
+  character(len=200) :: history_file = 'ccsm_output.nc'
+  logical :: mesh_from_file = .false.
 
   integer :: total_run_time, nsteps
   integer :: steps_per_output = 0
@@ -51,7 +59,7 @@ module ccs_params_mod
   namelist /domain/ mesh_file, xhalo, yhalo
   namelist /dynamic_opt/ reconstruct_scheme
   namelist /gpu_opt/ nx, precision, cfl, seed, n_bumps, mountain, u0, h_mean, radius, gravity, omega, device, &
-                     use_graph, final_state_file
+                     use_graph, final_state_file, history_file
 
 contains
 
@@ -82,7 +90,9 @@ contains
     if (trim(adjustl(reconstruct_scheme)) /= 'WENO') error stop 'Only reconstruct_scheme = WENO is implemented'
     if (xhalo /= yhalo) error stop 'xhalo and yhalo must be equal'
     if (xhalo < 3) error stop 'xhalo >= 3 is required (WENO-5 faces and the 6th-order vorticity)'
-    if (nx < 4 .or. mod(nx, 2) /= 0) error stop 'set nx (even, >= 4) in &gpu_opt: the netCDF mesh file cannot be read'
+    inquire(file=trim(mesh_file), exist=mesh_from_file)
+    if (.not. mesh_from_file .and. (nx < 4 .or. mod(nx, 2) /= 0)) &
+      error stop 'no mesh file: set nx (even, >= 4) in &gpu_opt to generate the mesh'
     if (history_interval <= 0) error stop 'history_interval must be positive'
     nsteps = int(total_run_time / dt)
   end subroutine initParameters
@@ -104,7 +114,8 @@ program ccsm
   type(ccs_metrics) :: m
   type(ccs_case_params) :: cp
   type(ccs_config) :: cfg
-  type(c_ptr) :: hnd
+  type(c_ptr) :: hnd, hist
+  type(ccs_history_desc) :: hd
   real(8) :: mass0, energy0, mass, energy, dummy, loop_s, area_sum
   character(len=*), parameter :: fmt_out = '(1x,a,i8,a,i8,2es25.16)'
 
@@ -112,14 +123,27 @@ program ccsm
   print *, 'Start the MCV Shallow Water model (MI355X, ', c_to_f_string(ccs_version()), ')'
   call initParameters
 
-  ! ---- read_mesh (io_mod.F90:15-307): generated conformal mesh + static tables
-  n = nx
+  ! ---- read_mesh (io_mod.F90:15-307): mesh file or generated conformal mesh,
+  ! then the static tables
+  if (mesh_from_file) then
+    call ccs_check(ccs_mesh_file_read(trim(mesh_file) // c_null_char, n, c_null_ptr, c_null_ptr, c_null_ptr), &
+                   'ccs_mesh_file_read ' // trim(mesh_file))
+    if (nx /= 0 .and. nx /= n) error stop 'nx in &gpu_opt differs from the mesh file'
+  else
+    n = nx
+  end if
   h = xhalo
   pw = n + 2 * h
   np = pw * pw * 6
   ni = n * n * 6
   allocate(lon_deg(ni), lat_deg(ni), xyz(3 * ni), jab(6 * ni))
-  call ccs_check(ccs_mesh_geometry(n, lon_deg, lat_deg, xyz, jab), 'ccs_mesh_geometry')
+  if (mesh_from_file) then
+    call ccs_check(ccs_mesh_file_read(trim(mesh_file) // c_null_char, n, c_loc(lon_deg), c_loc(lat_deg), &
+                                      c_loc(jab)), 'ccs_mesh_file_read ' // trim(mesh_file))
+    print *, 'Mesh read from ', trim(mesh_file), ': C', n
+  else
+    call ccs_check(ccs_mesh_geometry(n, lon_deg, lat_deg, xyz, jab), 'ccs_mesh_geometry')
+  end if
   allocate(lon(np), lat(np), sinlon(np), coslon(np), sinlat(np), coslat(np), sqrtG(np), f(np), area(np))
   allocate(G(4 * np), IG(4 * np), A(4 * np), IA(4 * np))
   m%n = n
@@ -192,6 +216,19 @@ program ccsm
   output_idx = 1
   total_output_num = int(nsteps * dt / history_interval)
   if (steps_per_output > 0) total_output_num = nsteps / steps_per_output
+  hist = c_null_ptr
+  if (len_trim(history_file) > 0) then
+    ! history_init + history_write_stat(stat(old), 1) (ccsm.F90:42-43)
+    hd%n = n
+    hd%halo = h
+    hd%case_num = case_num
+    hd%pad_ = 0
+    hd%dx_deg = 90.d0 / n
+    hd%dt = dt
+    call ccs_check(ccs_history_create(trim(history_file) // c_null_char, hd, lon, lat, area, phis, hist), &
+                   'ccs_history_create ' // trim(history_file))
+    call ccs_check(ccs_history_write(hist, 1, phi, phis, u, v, uc, vc, zonal, merid), 'ccs_history_write')
+  end if
   call ccs_check(ccs_diagnostics(hnd, mass0, energy0), 'ccs_diagnostics', hnd)
   write(*, fmt_out) 'output index/total, MCR, ECR :', output_idx - 1, '/', total_output_num, 0.d0, 0.d0
 
@@ -214,6 +251,11 @@ program ccsm
       call ccs_check(ccs_step(hnd, 1), 'ccs_step', hnd)
       call ccs_check(ccs_diagnostics(hnd, mass, dummy), 'ccs_diagnostics', hnd)
       output_idx = output_idx + 1
+      if (c_associated(hist)) then
+        call ccs_check(ccs_get_state(hnd, phiG, u, v, phi, uc, vc, zonal, merid), 'ccs_get_state', hnd)
+        call ccs_check(ccs_history_write(hist, output_idx, phi, phis, u, v, uc, vc, zonal, merid), &
+                       'ccs_history_write')
+      end if
       write(*, fmt_out) 'output index/total, MCR, ECR :', output_idx - 1, '/', total_output_num, &
                (mass - mass0) / mass0, (energy - energy0) / energy0
     else
@@ -234,6 +276,7 @@ program ccsm
     close(2)
   end if
   call ccs_destroy(hnd)
+  if (c_associated(hist)) call ccs_check(ccs_history_close(hist), 'ccs_history_close')
 
   call system_clock(t_end)
   print *, 'It took ', dble(t_end - t_start) / dble(rate), ' seconds to run this program'

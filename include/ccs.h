@@ -191,6 +191,43 @@ void ccs_destroy(ccs_handle* h);
 const char* ccs_last_error(const ccs_handle* h);
 const char* ccs_version(void);
 
+/* ------------------------------------------------------------ file formats */
+/* Mesh file in the schema of the reference mesh generator
+ * (mesh_generation/src/output_mod.f90:25-83), read by the dycore's read_mesh
+ * (io_mod.F90:37-117): the conformal map at the (2n+1)^2 extended points of
+ * each panel (cell centres at even indices), xi/eta, x/y/z (unit sphere),
+ * lon/lat (degrees), jab (per radian), attributes dx, dy, ids..ife.  Written
+ * as 64-bit-offset netCDF (the reference writes netCDF-4; nf90_open reads
+ * both).  Replaces the generator program ccs.exe. */
+int ccs_mesh_file_write(const char* path, int n);
+/* Reads a mesh file of that schema (classic or 64-bit-offset netCDF; not
+ * netCDF-4/HDF5): n from ids/ide (io_mod.F90:49-52) and the cell-centre
+ * lon/lat (degrees) and jab (io_mod.F90:122-135), laid out as
+ * ccs_mesh_geometry's outputs.  Output arrays may be NULL (n only). */
+int ccs_mesh_file_read(const char* path, int* n, double* lon_deg,
+                       double* lat_deg, double* jab);
+
+/* History file ccsm_output.nc of io_mod.F90:309-541 (64-bit-offset netCDF,
+ * as there): lon/lat (degrees, corners _FillValue), areaCell and phis on the
+ * padded grid; per output slot the time index and phi, phit = phi + phis, u,
+ * v, uc, vc, zonal_wind, meridional_wind (padded arrays).  Replaces
+ * history_init / history_write_stat. */
+typedef struct ccs_history ccs_history;
+typedef struct ccs_history_desc {
+  int n, halo, case_num, pad_;
+  double dx_deg;         /* grid spacing in degrees (90/n)                    */
+  double dt;             /* time step, seconds                                 */
+} ccs_history_desc;
+int ccs_history_create(const char* path, const ccs_history_desc* d,
+                       const double* lon, const double* lat,
+                       const double* areaCell, const double* phis,
+                       ccs_history** out);
+int ccs_history_write(ccs_history* h, int time_slot, const double* phi,
+                      const double* phis, const double* u, const double* v,
+                      const double* uc, const double* vc, const double* zonal,
+                      const double* merid);
+int ccs_history_close(ccs_history* h);
+
 /* ------------------------------------------------------- batched ensemble */
 /* Many independent runs, each with its own resolution, mesh, initial state
  * and time step, advanced together: one kernel launch per RK stage covers the

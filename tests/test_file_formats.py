@@ -1,0 +1,102 @@
+"""The reference's on-disk formats written natively (CPU): the mesh file of
+the generator (mesh_generation/src/output_mod.f90:25-83, read by
+io_mod.F90:37-117) and the history file (io_mod.F90:309-541), checked with an
+independent netCDF reader (scipy.io.netcdf_file)."""
+import math
+
+import numpy as np
+import pytest
+
+netcdf = pytest.importorskip("scipy.io").netcdf_file
+
+
+def test_mesh_file_schema_and_centres(tmp_path, ccs):
+    n = 16
+    path = str(tmp_path / "mesh_C16.nc")
+    ccs.write_mesh_file(path, n)
+    m = 2 * n + 1
+    with netcdf(path, "r", mmap=False) as f:
+        assert f.version_byte == 2
+        assert {k: f.dimensions[k] for k in ("three", "two", "lon", "lat", "nPatch")} == \
+            {"three": 3, "two": 2, "lon": m, "lat": m, "nPatch": 6}
+        assert f.dx == pytest.approx(90.0 / n) and f.dy == pytest.approx(90.0 / n)
+        assert (int(f.ids), int(f.ide), int(f.jds), int(f.jde), int(f.ifs), int(f.ife)) == (1, m, 1, m, 1, 6)
+        v = {k: f.variables[k][:].copy() for k in ("xi", "eta", "x", "y", "z", "lon", "lat", "jab")}
+        assert f.variables["lon"].units == b"degree_east"
+    assert v["lon"].shape == (6, m, m) and v["jab"].shape == (6, m, m, 2, 3)
+    # xi, eta: ((i-1) dx/2 + x_min) * D2M  (mesh_generation/src/mesh_mod.f90:47-48)
+    k = np.arange(m)
+    assert np.allclose(v["xi"][0, 0, :], -1 + k / n, atol=1e-15)
+    assert np.allclose(v["eta"][0, :, 0], -1 + k / n, atol=1e-15)
+    # every extended point is on the unit sphere, edges and corners included
+    r = np.sqrt(v["x"] ** 2 + v["y"] ** 2 + v["z"] ** 2)
+    assert np.max(np.abs(r - 1)) < 1e-14
+    # the dycore keeps the even (1-based) points: they are the cell centres
+    lon_c, lat_c, xyz_c, jab_c = (np.zeros(6 * n * n), np.zeros(6 * n * n), np.zeros(18 * n * n),
+                                  np.zeros(36 * n * n))
+    from conformal_cubed_sphere_amd import _lib
+    assert _lib.lib.ccs_mesh_geometry(n, _lib.dptr(lon_c), _lib.dptr(lat_c), _lib.dptr(xyz_c), _lib.dptr(jab_c)) == 0
+    cen = (slice(None), slice(1, m, 2), slice(1, m, 2))
+    assert np.max(np.abs(v["lon"][cen].ravel() - lon_c)) < 1e-12
+    assert np.max(np.abs(v["lat"][cen].ravel() - lat_c)) < 1e-12
+    assert np.max(np.abs(v["jab"][cen].reshape(-1) - jab_c)) < 1e-12
+    # shared panel edges: panel 0's east edge is panel 1's west edge (same points)
+    p0e = np.stack([v[c][0, :, m - 1] for c in ("x", "y", "z")], -1)
+    p1w = np.stack([v[c][1, :, 0] for c in ("x", "y", "z")], -1)
+    assert np.max(np.abs(p0e - p1w)) < 1e-13
+    # cube vertex: the map is singular there, its Jacobian vanishes; a vertex
+    # is the point with all |x| = |y| = |z| = 1/sqrt(3)
+    assert np.allclose(np.abs([v["x"][0, m - 1, m - 1], v["y"][0, m - 1, m - 1], v["z"][0, m - 1, m - 1]]),
+                       1 / math.sqrt(3), atol=1e-15)
+    assert np.max(np.abs(v["jab"][0, m - 1, m - 1])) == 0.0
+
+
+def test_mesh_file_reader_round_trip(tmp_path, ccs):
+    n = 12
+    path = str(tmp_path / "mesh.nc")
+    ccs.write_mesh_file(path, n)
+    n2, lon, lat, jab = ccs.read_mesh_file(path)
+    m = ccs.Mesh(n, halo=3)
+    assert n2 == n
+    # the extended-point solve converges to the same map as the cell-centre one
+    # to round-off (different Newton starting points)
+    assert np.max(np.abs(lon - m.lon_deg)) < 1e-11 and np.max(np.abs(lat - m.lat_deg)) < 1e-11
+    assert np.max(np.abs(jab - m.jab_unit)) < 1e-12
+
+
+def test_mesh_file_reader_rejects_non_classic_files(tmp_path, ccs):
+    bad = tmp_path / "hdf5.nc"
+    bad.write_bytes(b"\x89HDF\r\n\x1a\n" + bytes(64))
+    with pytest.raises(ccs.CcsError):
+        ccs.read_mesh_file(str(bad))
+
+
+def test_history_file_schema_and_records(tmp_path, ccs):
+    mesh = ccs.Mesh(8, halo=3)
+    s = ccs.init_case(mesh, case_num=7, seed=2, mountain=True)
+    path = str(tmp_path / "ccsm_output.nc")
+    with ccs.History(path, mesh, s.phis, dt=300.0, case_num=7) as h:
+        h.write(1, s)
+        s2 = s.copy()
+        s2.phi *= 1.001
+        h.write(2, s2)
+    P = mesh.P
+    with netcdf(path, "r", mmap=False) as f:
+        assert f.version_byte == 2
+        assert f.dimensions["lon"] == P and f.dimensions["lat"] == P and f.dimensions["nPatch"] == 6
+        assert f.dimensions["time"] is None  # unlimited
+        assert int(f.MCV_ORDER) == 3 and int(f.xhalo) == 3 and int(f.case_num) == 7 and float(f.dt) == 300.0
+        assert (int(f.ics), int(f.ice), int(f.its), int(f.ite)) == (-2, 11, 1, 8)
+        assert list(f.variables["time"][:]) == [1, 2]
+        phi = f.variables["phi"][:].copy()
+        phit = f.variables["phit"][:].copy()
+        lon = f.variables["lon"][:].copy()
+        zonal = f.variables["zonal_wind"][:].copy()
+        assert f.variables["phi"]._FillValue == -9999999999999999.0
+    assert phi.shape == (2, 6, P, P)
+    assert np.array_equal(phi[0].ravel(), s.phi) and np.array_equal(phi[1].ravel(), s2.phi)
+    assert np.array_equal(phit[1].ravel(), s2.phi + s.phis)
+    assert np.array_equal(zonal[0].ravel(), s.zonal)
+    corner = mesh.corner_mask().reshape(6, P, P)
+    assert np.all(lon[corner] == -9999999999999999.0)
+    assert np.allclose(lon[~corner].ravel(), np.degrees(mesh.lon.reshape(6, P, P)[~corner]))

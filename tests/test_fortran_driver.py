@@ -125,3 +125,33 @@ def test_driver_run_matches_oracle_bitwise(tmp_path, ccs, orc, scheme, code):
         assert np.array_equal(fin[k][mask], o.a[k][mask]), k
     mcr = [float(x.split()[-2]) for x in out.splitlines() if "MCR, ECR" in x]
     assert len(mcr) == 7 and max(abs(x) for x in mcr) < 1e-13
+    # history file (io_mod.F90:309-541): initial state + 6 outputs, the last
+    # one being the final state
+    from scipy.io import netcdf_file
+
+    with netcdf_file(str(tmp_path / "ccsm_output.nc"), "r", mmap=False) as f:
+        assert list(f.variables["time"][:]) == list(range(1, 8))
+        phi_last = f.variables["phi"][-1].copy().ravel()
+        phi_first = f.variables["phi"][0].copy().ravel()
+    assert np.array_equal(phi_last, fin["phi"])
+    assert np.array_equal(phi_first[mask], s0.phi[mask])
+
+
+def test_driver_reads_a_mesh_file(tmp_path, ccs):
+    """With &domain mesh_file naming a file of the generator's schema, the
+    driver takes the grid from it (io_mod.F90:37-135) instead of nx."""
+    _need_exe()
+    ccs.write_mesh_file(str(tmp_path / "mesh_C16.nc"), 16)
+    (tmp_path / "namelist.input").write_text(NAMELIST.format(
+        dt=600.0, hours=1, hist=3600, scheme="RK3_TVD", case=2, halo=3, gpu=" precision = 'fp64'").replace(
+        "unused.nc", "mesh_C16.nc"))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    out = r.stdout + r.stderr
+    assert "Mesh read from mesh_C16.nc" in out, out
+    m = ccs.Mesh(16, halo=3)
+    area = re.search(r"Spherical area, numerical, analytical :\s*(\S+)\s+(\S+)", out)
+    assert area and abs(float(area.group(1)) / float(np.sum(m.interior(m.areaCell))) - 1) < 1e-12, out
+    if "ccs error in ccs_create" in out:
+        assert r.returncode == 2
+    else:
+        assert r.returncode == 0, out
