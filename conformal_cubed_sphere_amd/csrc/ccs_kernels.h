@@ -63,6 +63,7 @@ struct StageArgsF64 {
   const double* tacc[9];    // kRK4f: L1, L2, L3 (phiG,u,v each)
   int n, h;
   double dx, dt, inc;
+  double rdx, r60dx;        // RN(1/dx), RN(1/(60 dx)); set by launch_stage_f64
 };
 
 void launch_stage_f64(hipStream_t s, int kind, bool explicit_derived, const StageArgsF64& a,

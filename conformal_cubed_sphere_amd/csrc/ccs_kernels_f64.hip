@@ -37,22 +37,34 @@ __device__ __forceinline__ long gidx(int i, int j, int p, int h, int P) {
   return static_cast<long>(i + h) + static_cast<long>(P) * (static_cast<long>(j + h) + static_cast<long>(P) * p);
 }
 
+// RN(a / b) for a divisor b whose correctly rounded reciprocal y = RN(1/b)
+// is known: q = RN(a y), r = a - b q (exact through the FMA), RN(q + r y).
+// This is Markstein's correction step; with y correctly rounded it returns
+// the correctly rounded quotient (bit-identical to the IEEE division) for
+// operands whose quotient is normal, in 3 instructions instead of the ~11 of
+// the general FP64 division sequence.
+__device__ __forceinline__ double div_known(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-q, b, a);
+  return fma(r, y, q);
+}
+
 // WENO_limiter with dir > 0 on the window (q1..q5) centred on q3
 // (spatial_operators_mod.F90:282-361).  dir < 0 is the same expression on the
 // reversed window, weno5(q5,q4,q3,q2,q1).
 __device__ __forceinline__ double weno5(double q1, double q2, double q3, double q4, double q5) {
-  const double s1 = q1 / 3. - 7. / 6. * q2 + 11. / 6. * q3;
-  const double s2 = -(q2 / 6.) + 5. / 6. * q3 + 1. / 3. * q4;
-  const double s3 = q3 / 3. + 5. / 6. * q4 - 1. / 6. * q5;
+  const double s1 = div_known(q1, 3., 1. / 3.) - 7. / 6. * q2 + 11. / 6. * q3;
+  const double s2 = -div_known(q2, 6., 1. / 6.) + 5. / 6. * q3 + 1. / 3. * q4;
+  const double s3 = div_known(q3, 3., 1. / 3.) + 5. / 6. * q4 - 1. / 6. * q5;
   const double a1 = q1 - 2. * q2 + q3;
   const double a2 = q2 - 2. * q3 + q4;
   const double a3 = q3 - 2. * q4 + q5;
   const double b1 = q1 - 4. * q2 + 3. * q3;
   const double b2 = q2 - q4;
   const double b3 = 3. * q3 - 4. * q4 + q5;
-  const double be1 = a1 * a1 * 13. / 12. + b1 * b1 * 0.25;
-  const double be2 = a2 * a2 * 13. / 12. + b2 * b2 * 0.25;
-  const double be3 = a3 * a3 * 13. / 12. + b3 * b3 * 0.25;
+  const double be1 = div_known(a1 * a1 * 13., 12., 1. / 12.) + b1 * b1 * 0.25;
+  const double be2 = div_known(a2 * a2 * 13., 12., 1. / 12.) + b2 * b2 * 0.25;
+  const double be3 = div_known(a3 * a3 * 13., 12., 1. / 12.) + b3 * b3 * 0.25;
   const double t40 = fabs(be1 - be2);
   const double t41 = fabs(be2 - be3);
   const double t5 = fabs(be3 - be1);
@@ -69,23 +81,44 @@ __device__ __forceinline__ double weno5(double q1, double q2, double q3, double 
     const double al2 = 0.6 * (1. + t5 / (1.E-2 + be2));
     const double al3 = 0.3 * (1. + t5 / (1.E-2 + be3));
     const double sa = al1 + al2 + al3;
-    w1 = al1 / sa; w2 = al2 / sa; w3 = al3 / sa;
+    // the three quotients by sa from its correctly rounded reciprocal
+    const double rsa = 1. / sa;
+    w1 = div_known(al1, sa, rsa); w2 = div_known(al2, sa, rsa); w3 = div_known(al3, sa, rsa);
   }
   return s1 * w1 + s2 * w2 + s3 * w3;
 }
 
 __device__ __forceinline__ double d6(double m3, double m2, double m1, double p1, double p2, double p3,
-                                     double dh) {
-  return (-m3 + 9. * m2 - 45. * m1 + 45. * p1 - 9. * p2 + p3) / (60. * dh);
+                                     double dh, double r60dh) {
+  return div_known(-m3 + 9. * m2 - 45. * m1 + 45. * p1 - 9. * p2 + p3, 60. * dh, r60dh);
 }
 
 template <int KIND, bool EXPL>
 __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, HaloScatter hs) {
-  // LDS fields on the extended tile (row-major [lj][li])
-  __shared__ double sFx[NE], sFy[NE], sE[NE], sPtG[NE], sUC[NE], sVC[NE], sPt[NE], sG11[NE],
-      sG22[NE], sU[NE], sV[NE];
-  __shared__ double sFxF[TY * (TX + 1)], sExF[TY * (TX + 1)];
-  __shared__ double sFyF[(TY + 1) * TX], sEyF[(TY + 1) * TX];
+  // LDS, sized so that four workgroups fit a CU.  Fields read along both
+  // directions cover the extended tile (row-major [lj][li], EX x EY); fields
+  // read only along x cover the tile's rows (XB: [lj-3][li], EX x TY), those
+  // read only along y the tile's columns (YB: [lj][li-3], TX x EY).  The face
+  // values are written after a barrier into the storage of sFx/sFy/sE, which
+  // phase 3 no longer reads.
+  constexpr int NXB = EX * TY, NYB = TX * EY;
+  constexpr int NFX = (TX + 1) * TY;
+  constexpr int NFY = TX * (TY + 1);
+  static_assert(2 * NFX + 2 * NFY <= NXB + NYB + NE, "face values fit the dead field storage");
+  static_assert((NT & (NT - 1)) == 0 && NT % 64 == 0, "the last face pass rotates whole waves");
+  __shared__ double s_xy[NXB + NYB + NE];  // sFx | sFy | sE, later the face values
+  __shared__ double sPtG[NE], sPt[NE];
+  __shared__ double sUC[NXB], sG11[NXB], sV[NXB];
+  __shared__ double sVC[NYB], sG22[NYB], sU[NYB];
+  double* const sFx = s_xy;
+  double* const sFy = s_xy + NXB;
+  double* const sE = s_xy + NXB + NYB;
+  double* const sFxF = s_xy;
+  double* const sExF = s_xy + NFX;
+  double* const sFyF = s_xy + 2 * NFX;
+  double* const sEyF = s_xy + 2 * NFX + NFY;
+  auto xb = [](int lj, int li) { return (lj - 3) * EX + li; };
+  auto yb = [](int lj, int li) { return lj * TX + li - 3; };
 
   const int n = a.n, h = a.h, P = n + 2 * h;
   // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD gets a
@@ -133,66 +166,87 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
     const double K = 0.5 * (u * uc + v * vc);
     const int l = lj * EX + li;
     sE[l] = phit + K;
-    sFx[l] = phiG * uc;
-    sFy[l] = phiG * vc;
     sPtG[l] = phitG;
     sPt[l] = phit;
-    sUC[l] = uc;
-    sVC[l] = vc;
-    sG11[l] = g11;
-    sG22[l] = g22;
-    sU[l] = u;
-    sV[l] = v;
+    if (lj >= 3 && lj < TY + 3) {  // tile rows: x-direction fields
+      const int lx = xb(lj, li);
+      sFx[lx] = phiG * uc;
+      sUC[lx] = uc;
+      sG11[lx] = g11;
+      sV[lx] = v;
+    }
+    if (li >= 3 && li < TX + 3) {  // tile columns: y-direction fields
+      const int ly = yb(lj, li);
+      sFy[ly] = phiG * vc;
+      sVC[ly] = vc;
+      sG22[ly] = g22;
+      sU[ly] = u;
+    }
   }
   __syncthreads();
 
-  // ---- phase 2: faces (spatial_operators_mod.F90:80-149)
-  constexpr int NFX = (TX + 1) * TY;
-  constexpr int NFY = TX * (TY + 1);
-  for (int e = tid; e < NFX + NFY; e += NT) {
+  // ---- phase 2: faces (spatial_operators_mod.F90:80-149), held in registers
+  // until every thread has finished reading the fields.  x and y faces run
+  // the same code on strided LDS views, so no wave executes both variants.
+  // Faces [2 NT, NFX + NFY) fill only part of a third pass; which wave takes
+  // them rotates with the workgroup so that the SIMDs share the extra work.
+  constexpr int NFACE_IT = (NFX + NFY + NT - 1) / NT;
+  double fF[NFACE_IT], fE[NFACE_IT];
+#pragma unroll
+  for (int it = 0; it < NFACE_IT; ++it) {
+    const int e = it * NT + ((it == NFACE_IT - 1) ? ((tid + 64 * static_cast<int>(b & 3u)) & (NT - 1)) : tid);
+    fF[it] = 0.;
+    fE[it] = 0.;
+    if (e >= NFX + NFY) continue;
+    // band offset/stride (fields read along one direction) and full-tile
+    // offset/stride of the face's left (lower) cell
+    int ob, sb, of, sf;
+    bool ok;
     if (e < NFX) {
       const int fi = e % (TX + 1), ty = e / (TX + 1);
       const int ig = i0 + fi - 1, j = j0 + ty;
-      if (j >= n || ig < -1 || ig > n - 1) continue;
-      const int r = (ty + 3) * EX;
-      const int c = fi + 2;  // LDS column of cell ig
-      const double* F = sFx + r;
-      const double* E = sE + r;
-      const double* Q = sPtG + r;
-      const double Fp = weno5(F[c - 2], F[c - 1], F[c], F[c + 1], F[c + 2]);
-      const double Fn = weno5(F[c + 3], F[c + 2], F[c + 1], F[c], F[c - 1]);
-      const double Ep = weno5(E[c - 2], E[c - 1], E[c], E[c + 1], E[c + 2]);
-      const double En = weno5(E[c + 3], E[c + 2], E[c + 1], E[c], E[c - 1]);
-      const double Pp = weno5(Q[c - 2], Q[c - 1], Q[c], Q[c + 1], Q[c + 2]);
-      const double Pn = weno5(Q[c + 3], Q[c + 2], Q[c + 1], Q[c], Q[c - 1]);
-      const double uB = 0.5 * (sUC[r + c + 1] + sUC[r + c]);
-      const double pB = 0.5 * (sPt[r + c + 1] + sPt[r + c]);
-      const double gq = 0.5 * (sG11[r + c + 1] + sG11[r + c]) * pB;
-      const double e1 = uB - sqrt(gq), e2 = uB + sqrt(gq);
-      const double lam = fabs(e1) > fabs(e2) ? fabs(e1) : fabs(e2);
-      sFxF[ty * (TX + 1) + fi] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
-      sExF[ty * (TX + 1) + fi] = 0.5 * (Ep + En - lam * (0. - 0.));
+      ok = j < n && ig >= -1 && ig <= n - 1;
+      ob = ty * EX + fi + 2; sb = 1;           // x band: row ty of the tile
+      of = (ty + 3) * EX + fi + 2; sf = 1;
     } else {
       const int e2 = e - NFX;
       const int tx = e2 % TX, fj = e2 / TX;
       const int i = i0 + tx, jg = j0 + fj - 1;
-      if (i >= n || jg < -1 || jg > n - 1) continue;
-      const int c = tx + 3;
-      const int r = fj + 2;  // LDS row of cell jg
-      auto at = [c](const double* s, int row) { return s[row * EX + c]; };
-      const double Fp = weno5(at(sFy, r - 2), at(sFy, r - 1), at(sFy, r), at(sFy, r + 1), at(sFy, r + 2));
-      const double Fn = weno5(at(sFy, r + 3), at(sFy, r + 2), at(sFy, r + 1), at(sFy, r), at(sFy, r - 1));
-      const double Ep = weno5(at(sE, r - 2), at(sE, r - 1), at(sE, r), at(sE, r + 1), at(sE, r + 2));
-      const double En = weno5(at(sE, r + 3), at(sE, r + 2), at(sE, r + 1), at(sE, r), at(sE, r - 1));
-      const double Pp = weno5(at(sPtG, r - 2), at(sPtG, r - 1), at(sPtG, r), at(sPtG, r + 1), at(sPtG, r + 2));
-      const double Pn = weno5(at(sPtG, r + 3), at(sPtG, r + 2), at(sPtG, r + 1), at(sPtG, r), at(sPtG, r - 1));
-      const double vB = 0.5 * (at(sVC, r + 1) + at(sVC, r));
-      const double pB = 0.5 * (at(sPt, r + 1) + at(sPt, r));
-      const double gq = 0.5 * (at(sG22, r + 1) + at(sG22, r)) * pB;
-      const double e1 = vB - sqrt(gq), e2v = vB + sqrt(gq);
-      const double lam = fabs(e1) > fabs(e2v) ? fabs(e1) : fabs(e2v);
-      sFyF[fj * TX + tx] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
-      sEyF[fj * TX + tx] = 0.5 * (Ep + En - lam * (0. - 0.));
+      ok = i < n && jg >= -1 && jg <= n - 1;
+      ob = (fj + 2) * TX + tx; sb = TX;        // y band: column tx of the tile
+      of = (fj + 2) * EX + tx + 3; sf = EX;
+    }
+    if (!ok) continue;
+    const double* F = (e < NFX ? sFx : sFy) + ob;
+    const double* U = (e < NFX ? sUC : sVC) + ob;
+    const double* G = (e < NFX ? sG11 : sG22) + ob;
+    const double* E = sE + of;
+    const double* Q = sPtG + of;
+    const double* Pt = sPt + of;
+    const double Fp = weno5(F[-2 * sb], F[-sb], F[0], F[sb], F[2 * sb]);
+    const double Fn = weno5(F[3 * sb], F[2 * sb], F[sb], F[0], F[-sb]);
+    const double Ep = weno5(E[-2 * sf], E[-sf], E[0], E[sf], E[2 * sf]);
+    const double En = weno5(E[3 * sf], E[2 * sf], E[sf], E[0], E[-sf]);
+    const double Pp = weno5(Q[-2 * sf], Q[-sf], Q[0], Q[sf], Q[2 * sf]);
+    const double Pn = weno5(Q[3 * sf], Q[2 * sf], Q[sf], Q[0], Q[-sf]);
+    const double uB = 0.5 * (U[sb] + U[0]);
+    const double pB = 0.5 * (Pt[sf] + Pt[0]);
+    const double gq = 0.5 * (G[sb] + G[0]) * pB;
+    const double e1 = uB - sqrt(gq), e2 = uB + sqrt(gq);
+    const double lam = fabs(e1) > fabs(e2) ? fabs(e1) : fabs(e2);
+    fF[it] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
+    fE[it] = 0.5 * (Ep + En - lam * (0. - 0.));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NFACE_IT; ++it) {
+    const int e = it * NT + ((it == NFACE_IT - 1) ? ((tid + 64 * static_cast<int>(b & 3u)) & (NT - 1)) : tid);
+    if (e < NFX) {
+      sFxF[e] = fF[it];  // e = ty * (TX + 1) + fi
+      sExF[e] = fE[it];
+    } else if (e < NFX + NFY) {
+      sFyF[e - NFX] = fF[it];  // fj * TX + tx
+      sEyF[e - NFX] = fE[it];
     }
   }
   __syncthreads();
@@ -202,19 +256,21 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
   const int i = i0 + tx, j = j0 + ty;
   if (i >= n || j >= n) return;
   const double dx = a.dx;
-  const int lc = (ty + 3) * EX + (tx + 3);
-  const double dfluxdx = (sFxF[ty * (TX + 1) + tx + 1] - sFxF[ty * (TX + 1) + tx]) / dx;
-  const double dEdx = (sExF[ty * (TX + 1) + tx + 1] - sExF[ty * (TX + 1) + tx]) / dx;
-  const double dvdx = d6(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], dx);
-  const double dfluxdy = (sFyF[(ty + 1) * TX + tx] - sFyF[ty * TX + tx]) / dx;
-  const double dEdy = (sEyF[(ty + 1) * TX + tx] - sEyF[ty * TX + tx]) / dx;
-  const double dudy = d6(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX],
-                         sU[lc + 3 * EX], dx);
+  const double rdx = a.rdx, r60 = a.r60dx;
+  const double dfluxdx = div_known(sFxF[ty * (TX + 1) + tx + 1] - sFxF[ty * (TX + 1) + tx], dx, rdx);
+  const double dEdx = div_known(sExF[ty * (TX + 1) + tx + 1] - sExF[ty * (TX + 1) + tx], dx, rdx);
+  const int lxc = ty * EX + (tx + 3);   // x band, this cell
+  const int lyc = (ty + 3) * TX + tx;   // y band, this cell
+  const double dvdx = d6(sV[lxc - 3], sV[lxc - 2], sV[lxc - 1], sV[lxc + 1], sV[lxc + 2], sV[lxc + 3], dx, r60);
+  const double dfluxdy = div_known(sFyF[(ty + 1) * TX + tx] - sFyF[ty * TX + tx], dx, rdx);
+  const double dEdy = div_known(sEyF[(ty + 1) * TX + tx] - sEyF[ty * TX + tx], dx, rdx);
+  const double dudy = d6(sU[lyc - 3 * TX], sU[lyc - 2 * TX], sU[lyc - TX], sU[lyc + TX], sU[lyc + 2 * TX],
+                         sU[lyc + 3 * TX], dx, r60);
   const long g = gidx(i, j, p, h, P);
   const double vort = dvdx - dudy + st.sqrtG[g] * st.f[g];
   const double tp = -(dfluxdx + dfluxdy);
-  const double tu = -dEdx + vort * sVC[lc];
-  const double tv = -dEdy - vort * sUC[lc];
+  const double tu = -dEdx + vort * sVC[lyc];
+  const double tv = -dEdy - vort * sUC[lxc];
 
   const long ii = static_cast<long>(i) + static_cast<long>(n) * (static_cast<long>(j) + static_cast<long>(n) * p);
   if (KIND == kTendOnly) {
@@ -233,9 +289,9 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
     un = 0.75 * a.base.u[g] + 0.25 * a.in.u[g] + 0.25 * a.dt * tu;
     vn = 0.75 * a.base.v[g] + 0.25 * a.in.v[g] + 0.25 * a.dt * tv;
   } else if (KIND == kRK3s3) {
-    pn = a.base.phiG[g] / 3. + 2. / 3. * a.in.phiG[g] + 2. / 3. * a.dt * tp;
-    un = a.base.u[g] / 3. + 2. / 3. * a.in.u[g] + 2. / 3. * a.dt * tu;
-    vn = a.base.v[g] / 3. + 2. / 3. * a.in.v[g] + 2. / 3. * a.dt * tv;
+    pn = div_known(a.base.phiG[g], 3., 1. / 3.) + 2. / 3. * a.in.phiG[g] + 2. / 3. * a.dt * tp;
+    un = div_known(a.base.u[g], 3., 1. / 3.) + 2. / 3. * a.in.u[g] + 2. / 3. * a.dt * tu;
+    vn = div_known(a.base.v[g], 3., 1. / 3.) + 2. / 3. * a.in.v[g] + 2. / 3. * a.dt * tv;
   } else if (KIND == kRK4a) {
     a.tend[0][ii] = tp;
     a.tend[1][ii] = tu;
@@ -244,9 +300,9 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
     un = a.base.u[g] + a.inc * tu;
     vn = a.base.v[g] + a.inc * tv;
   } else {  // kRK4f
-    const double T0 = (a.tacc[0][ii] + 2. * a.tacc[3][ii] + 2. * a.tacc[6][ii] + tp) / 6.;
-    const double T1 = (a.tacc[1][ii] + 2. * a.tacc[4][ii] + 2. * a.tacc[7][ii] + tu) / 6.;
-    const double T2 = (a.tacc[2][ii] + 2. * a.tacc[5][ii] + 2. * a.tacc[8][ii] + tv) / 6.;
+    const double T0 = div_known(a.tacc[0][ii] + 2. * a.tacc[3][ii] + 2. * a.tacc[6][ii] + tp, 6., 1. / 6.);
+    const double T1 = div_known(a.tacc[1][ii] + 2. * a.tacc[4][ii] + 2. * a.tacc[7][ii] + tu, 6., 1. / 6.);
+    const double T2 = div_known(a.tacc[2][ii] + 2. * a.tacc[5][ii] + 2. * a.tacc[8][ii] + tv, 6., 1. / 6.);
     pn = a.base.phiG[g] + a.dt * T0;
     un = a.base.u[g] + a.dt * T1;
     vn = a.base.v[g] + a.dt * T2;
@@ -392,8 +448,11 @@ __global__ void __launch_bounds__(kDiagThreads) diag_final_f64(const double* par
 
 }  // namespace
 
-void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a, const StaticsF64& st,
+void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_in, const StaticsF64& st,
                       const HaloScatter& hs) {
+  StageArgsF64 a = a_in;
+  a.rdx = 1.0 / a.dx;            // correctly rounded reciprocals for div_known
+  a.r60dx = 1.0 / (60.0 * a.dx);
   const dim3 grid(static_cast<unsigned>(((a.n + TX - 1) / TX) * ((a.n + TY - 1) / TY) * 6));
   const dim3 block(NT);
 #define CCS_LAUNCH(K)                                                                   \
