@@ -54,11 +54,14 @@ def parse():
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=60)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--spatial-op", default="weno",
+                    choices=["weno", "centred2", "centred4", "centred6", "weno_ext"],
+                    help="spatial_operators_mod variant (runs on the FP64 parity path)")
     return ap.parse_args()
 
 
-def time_path(ccs, mesh, s0, dt, precision, steps, warmup):
-    core = ccs.DynamicalCore(mesh, dt, precision=precision, use_graph=True)
+def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno"):
+    core = ccs.DynamicalCore(mesh, dt, precision=precision, use_graph=True, spatial_op=spatial_op)
     core.set_state(s0)
     core.step(1)          # first step reads the uploaded companions
     core.step(warmup)
@@ -187,6 +190,8 @@ def main():
     cells = 6 * args.n * args.n
 
     headline = args.precision
+    if args.spatial_op != "weno":
+        headline = "fp64"  # the operator variants run on the FP64 parity path
     if headline == "auto":
         headline = "fp32"
         try:
@@ -194,7 +199,8 @@ def main():
         except ccs.CcsError:
             headline = "fp64"
     dtype = "f32" if headline == "fp32" else "f64"
-    core, wall, ev_total, kern_ms = time_path(ccs, mesh, s0, dt, headline, args.steps, args.warmup)
+    core, wall, ev_total, kern_ms = time_path(ccs, mesh, s0, dt, headline, args.steps, args.warmup,
+                                              args.spatial_op)
     value = cells * args.steps / wall
     bpl = alg_bytes_per_launch(headline)
     achieved = bpl * cells / (kern_ms * 1e-3) / 1e9  # GB/s of the stage kernel
@@ -203,9 +209,10 @@ def main():
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
         "config": {"workload": f"C{args.n} seeded balanced jet + 8 bumps + Gaussian mountain, SSP-RK3, "
-                               f"CFL 0.5, halo 3, precision {headline}",
+                               f"CFL 0.5, halo 3, precision {headline}"
+                               + ("" if args.spatial_op == "weno" else f", spatial operator {args.spatial_op}"),
                    "n": args.n, "halo": 3, "dt_s": dt, "steps_per_5_days": nsteps_5d,
-                   "interior_cells": cells, "precision": headline},
+                   "interior_cells": cells, "precision": headline, "spatial_op": args.spatial_op},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "stage kernel (fused tendency + RK stage + halo exchange)",
@@ -214,7 +221,7 @@ def main():
         "event_ms_per_step": ev_total / args.steps,
     }
     prof = os.path.join(ROOT, "profiles", f"traffic_c{args.n}_{headline}.json")
-    if os.path.exists(prof):
+    if os.path.exists(prof) and args.spatial_op == "weno":
         with open(prof) as f:
             t = json.load(f)
         out["roofline"]["traffic"] = t.get("hbm_bytes_per_launch")

@@ -18,9 +18,11 @@ from .core import (  # noqa: F401
     cfl_time_step,
     error_norms,
     init_case,
+    SPATIAL_OPS,
     read_mesh_file,
+    read_mesh_file_ext,
     write_mesh_file,
 )
 
-__all__ = ["CcsError", "DynamicalCore", "Ensemble", "History", "Mesh", "State", "cfl_time_step", "error_norms", "init_case", "read_mesh_file",
-           "write_mesh_file"]
+__all__ = ["CcsError", "DynamicalCore", "Ensemble", "History", "Mesh", "SPATIAL_OPS", "State", "cfl_time_step",
+           "error_norms", "init_case", "read_mesh_file", "read_mesh_file_ext", "write_mesh_file"]

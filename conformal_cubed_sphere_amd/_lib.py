@@ -29,6 +29,12 @@ CCS_FP32_SPLIT = 1
 CCS_FP64_FAST = 2
 CCS_RK3_TVD = 0
 CCS_RK4 = 1
+# enum ccs_spatial_op
+CCS_OP_WENO = 0
+CCS_OP_CENTRED2 = 1
+CCS_OP_CENTRED4 = 2
+CCS_OP_CENTRED6 = 3
+CCS_OP_WENO_EXT = 4
 
 STATUS_NAMES = {
     0: "CCS_OK", 1: "CCS_ERR_INVALID", 2: "CCS_ERR_DEVICE", 3: "CCS_ERR_ORDER",
@@ -59,7 +65,7 @@ class CcsConfig(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int), ("halo", ctypes.c_int), ("precision", ctypes.c_int),
         ("scheme", ctypes.c_int), ("eflux", ctypes.c_int), ("device", ctypes.c_int),
-        ("use_graph", ctypes.c_int), ("pad_", ctypes.c_int),
+        ("use_graph", ctypes.c_int), ("spatial_op", ctypes.c_int),
         ("dt", ctypes.c_double), ("dx", ctypes.c_double), ("radius", ctypes.c_double),
         ("gravity", ctypes.c_double), ("omega", ctypes.c_double),
     ]
@@ -85,11 +91,13 @@ _I = ctypes.POINTER(ctypes.c_int)
 SIGNATURES = {
     "ccs_mesh_geometry": (ctypes.c_int, [ctypes.c_int, _D, _D, _D, _D]),
     "ccs_mesh_metrics": (ctypes.c_int, [_D, _D, _D, ctypes.POINTER(CcsMetrics)]),
+    "ccs_mesh_ext_metric": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _D, _D]),
     "ccs_case_defaults": (None, [ctypes.POINTER(CcsCaseParams)]),
     "ccs_init_case": (ctypes.c_int, [ctypes.POINTER(CcsMetrics), ctypes.POINTER(CcsCaseParams),
                                      _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     "ccs_create": (ctypes.c_int, [ctypes.POINTER(CcsConfig), ctypes.POINTER(ctypes.c_void_p)]),
     "ccs_set_mesh": (ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, _D]),
+    "ccs_set_mesh_ext": (ctypes.c_int, [ctypes.c_void_p, _D]),
     "ccs_set_state": (ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D, _D]),
     "ccs_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ccs_get_state": (ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -102,6 +110,7 @@ SIGNATURES = {
     "ccs_version": (ctypes.c_char_p, []),
     "ccs_mesh_file_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "ccs_mesh_file_read": (ctypes.c_int, [ctypes.c_char_p, _I, _D, _D, _D]),
+    "ccs_mesh_file_read_ext": (ctypes.c_int, [ctypes.c_char_p, _I, _D]),
     "ccs_history_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CcsHistoryDesc), _D, _D, _D, _D,
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "ccs_history_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D]),

@@ -23,6 +23,10 @@ OMEGA = 7.292e-5         # constants_mod.F90:14
 
 PRECISIONS = {"fp64": _lib.CCS_FP64_PARITY, "fp32": _lib.CCS_FP32_SPLIT, "fp64_fast": _lib.CCS_FP64_FAST}
 SCHEMES = {"RK3_TVD": _lib.CCS_RK3_TVD, "RK4": _lib.CCS_RK4}
+# spatial_operators_mod variants (enum ccs_spatial_op): the reference's default
+# WENO-Z module and the swap-in sources _2nd, _4th, _6th and _WENO
+SPATIAL_OPS = {"weno": _lib.CCS_OP_WENO, "centred2": _lib.CCS_OP_CENTRED2, "centred4": _lib.CCS_OP_CENTRED4,
+               "centred6": _lib.CCS_OP_CENTRED6, "weno_ext": _lib.CCS_OP_WENO_EXT}
 
 
 class CcsError(RuntimeError):
@@ -85,6 +89,17 @@ class Mesh:
         idx = np.arange(self.P) - self.halo
         out = (idx < 0) | (idx >= self.n)
         return np.broadcast_to(out[None, :, None] & out[None, None, :], (6, self.P, self.P))
+
+    def ig_ext(self, jab_ext=None):
+        """mesh%matrixIG_ext (io_mod.F90:223-250): G^-1 on the (2n+1)^2
+        extended lattice of each panel, (2,2,2n+1,2n+1,6) column-major, from
+        this library's conformal map or from a mesh file's jab
+        (read_mesh_file_ext)."""
+        m = 2 * self.n + 1
+        out = np.zeros(4 * m * m * 6)
+        src = None if jab_ext is None else np.ascontiguousarray(jab_ext, dtype=np.float64)
+        _check("ccs_mesh_ext_metric", lib.ccs_mesh_ext_metric(self.n, self.radius, dptr(src), dptr(out)))
+        return out
 
 
 @dataclass
@@ -172,12 +187,14 @@ class DynamicalCore:
     """
 
     def __init__(self, mesh: Mesh, dt: float, precision: str = "fp64", scheme: str = "RK3_TVD",
-                 device: int = 0, use_graph: bool = True, gravity: float = GRAVITY):
+                 device: int = 0, use_graph: bool = True, gravity: float = GRAVITY, spatial_op: str = "weno",
+                 ig_ext=None):
         self.mesh = mesh
         cfg = _lib.CcsConfig()
         cfg.n, cfg.halo = mesh.n, mesh.halo
         cfg.precision = PRECISIONS[precision]
         cfg.scheme = SCHEMES[scheme]
+        cfg.spatial_op = SPATIAL_OPS[spatial_op]
         cfg.eflux, cfg.device, cfg.use_graph = 0, int(device), int(bool(use_graph))
         cfg.dt, cfg.dx, cfg.radius, cfg.gravity, cfg.omega = float(dt), mesh.dx, mesh.radius, gravity, mesh.omega
         self.cfg = cfg
@@ -186,6 +203,10 @@ class DynamicalCore:
         _check("ccs_create", lib.ccs_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
         self._phis = None
+        self.spatial_op = spatial_op
+        if spatial_op == "weno_ext":
+            self._ig_ext = np.ascontiguousarray(mesh.ig_ext() if ig_ext is None else ig_ext, dtype=np.float64)
+            _check("ccs_set_mesh_ext", lib.ccs_set_mesh_ext(self._h, dptr(self._ig_ext)), self._h)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -366,6 +387,17 @@ def read_mesh_file(path: str):
     _check("ccs_mesh_file_read", lib.ccs_mesh_file_read(os.fsencode(path), ctypes.byref(n), dptr(lon), dptr(lat),
                                                         dptr(jab)))
     return n.value, lon, lat, jab
+
+
+def read_mesh_file_ext(path: str):
+    """(n, jab_ext): the whole jab variable of a mesh file, (3,2,2n+1,2n+1,6)
+    per radian, as io_mod.F90:107 reads it into mesh%jab_ext."""
+    n = ctypes.c_int()
+    _check("ccs_mesh_file_read_ext", lib.ccs_mesh_file_read_ext(os.fsencode(path), ctypes.byref(n), None))
+    m = 2 * n.value + 1
+    jab = np.zeros(36 * m * m)
+    _check("ccs_mesh_file_read_ext", lib.ccs_mesh_file_read_ext(os.fsencode(path), ctypes.byref(n), dptr(jab)))
+    return n.value, jab
 
 
 class History:

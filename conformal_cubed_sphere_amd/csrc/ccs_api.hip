@@ -32,7 +32,7 @@ struct ccs_handle {
   std::size_t np = 0, ni = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  bool mesh_set = false, state_set = false, explicit_valid = false;
+  bool mesh_set = false, state_set = false, explicit_valid = false, ext_set = false;
   HaloScatter hs{};
   // FP64 parity path
   double* dmem = nullptr;
@@ -43,6 +43,7 @@ struct ccs_handle {
   double* derived[5]{};    // get_state scratch: phi, uc, vc, zonal, merid
   double* partials = nullptr;
   double* result = nullptr;
+  double* face_speed = nullptr;  // CCS_OP_WENO_EXT: x-face then y-face G^-1 entries
   // split-precision path
   SplitState* split = nullptr;
   // graphs of one step ([0]) and of kGraphSteps steps ([1]); every step
@@ -74,10 +75,19 @@ int fail(ccs_handle* h, int code, const std::string& msg) {
 
 bool valid_precision(int p) { return p == CCS_FP64_PARITY || p == CCS_FP32_SPLIT || p == CCS_FP64_FAST; }
 
+void op_args(const ccs_handle* hd, StageArgsF64& a) {
+  a.op = hd->cfg.spatial_op;
+  if (hd->face_speed) {
+    a.igx = hd->face_speed;
+    a.igy = hd->face_speed + static_cast<std::size_t>(hd->n + 1) * hd->n * 6;
+  }
+}
+
 int enqueue_step_f64(ccs_handle* hd, bool expl, hipEvent_t* ev) {
   StageArgsF64 a{};
   a.n = hd->n;
   a.h = hd->h;
+  op_args(hd, a);
   a.dx = hd->cfg.dx;
   a.dt = hd->cfg.dt;
   a.phi0 = hd->phi0;
@@ -143,6 +153,9 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
   if (cfg->scheme != CCS_RK3_TVD && cfg->scheme != CCS_RK4) return CCS_ERR_INVALID;
   if (cfg->eflux != CCS_EFLUX_NONE) return CCS_ERR_INVALID;
   if (!(cfg->dt > 0.0) || !(cfg->dx > 0.0) || !(cfg->radius > 0.0)) return CCS_ERR_INVALID;
+  if (cfg->spatial_op < CCS_OP_WENO || cfg->spatial_op > CCS_OP_WENO_EXT) return CCS_ERR_INVALID;
+  // the split-precision kernels implement the reference's default operator only
+  if (cfg->spatial_op != CCS_OP_WENO && cfg->precision != CCS_FP64_PARITY) return CCS_ERR_INVALID;
   ccs_handle* hd = new ccs_handle();
   hd->cfg = *cfg;
   hd->n = cfg->n;
@@ -216,6 +229,7 @@ extern "C" void ccs_destroy(ccs_handle* hd) {
   }
   if (hd->stream) hipStreamSynchronize(hd->stream);
   if (hd->dmem) hipFree(hd->dmem);
+  if (hd->face_speed) hipFree(hd->face_speed);
   if (hd->split) split_destroy(hd->split);
   if (hd->stream) hipStreamDestroy(hd->stream);
   delete hd;
@@ -269,6 +283,40 @@ extern "C" int ccs_set_mesh(ccs_handle* hd, const double* sqrtG, const double* G
   return CCS_OK;
 }
 
+extern "C" int ccs_set_mesh_ext(ccs_handle* hd, const double* ig_ext) {
+  if (!hd) return CCS_ERR_INVALID;
+  if (!ig_ext) return fail(hd, CCS_ERR_INVALID, "ccs_set_mesh_ext: null array");
+  if (hd->cfg.spatial_op != CCS_OP_WENO_EXT)
+    return fail(hd, CCS_ERR_INVALID, "ccs_set_mesh_ext: only CCS_OP_WENO_EXT reads matrixIG_ext");
+  HIPCHK(hd, hipSetDevice(hd->cfg.device));
+  const int n = hd->n;
+  const std::size_t m = 2 * static_cast<std::size_t>(n) + 1, nf = static_cast<std::size_t>(n + 1) * n * 6;
+  // matrixIG_ext(r,c,a,b,p), 1-based lattice indices
+  auto ext = [&](int r, int c, int ia, int ib, int p) {
+    return ig_ext[(r - 1) + 2 * (c - 1) + 4 * ((ia - 1) + m * ((ib - 1) + m * (p - 1)))];
+  };
+  std::vector<double> t(2 * nf);
+  for (int p = 1; p <= 6; ++p)
+    for (int j = 1; j <= n; ++j)
+      for (int i = 0; i <= n; ++i)  // x-face (i+1/2, j): lattice (2i+1, j)
+        t[i + static_cast<std::size_t>(n + 1) * ((j - 1) + static_cast<std::size_t>(n) * (p - 1))] =
+            ext(1, 1, 2 * i + 1, j, p);
+  for (int p = 1; p <= 6; ++p)
+    for (int j = 0; j <= n; ++j)
+      for (int i = 1; i <= n; ++i)  // y-face (i, j+1/2): lattice (i, 2j+1)
+        t[nf + (i - 1) + static_cast<std::size_t>(n) * (j + static_cast<std::size_t>(n + 1) * (p - 1))] =
+            ext(2, 2, i, 2 * j + 1, p);
+  for (double x : t)
+    if (!std::isfinite(x) || x < 0.0)
+      return fail(hd, CCS_ERR_INVALID, "ccs_set_mesh_ext: face entries of matrixIG_ext must be finite and >= 0");
+  if (!hd->face_speed && hipMalloc(&hd->face_speed, 2 * nf * sizeof(double)) != hipSuccess)
+    return fail(hd, CCS_ERR_NOMEM, "ccs_set_mesh_ext: device allocation failed");
+  HIPCHK(hd, hipMemcpyAsync(hd->face_speed, t.data(), 2 * nf * sizeof(double), hipMemcpyHostToDevice, hd->stream));
+  HIPCHK(hd, hipStreamSynchronize(hd->stream));
+  hd->ext_set = true;
+  return CCS_OK;
+}
+
 extern "C" int ccs_set_state(ccs_handle* hd, const double* phiG, const double* u, const double* v,
                              const double* phi, const double* uc, const double* vc) {
   if (!hd) return CCS_ERR_INVALID;
@@ -305,6 +353,8 @@ extern "C" int ccs_step(ccs_handle* hd, int nsteps) {
   if (!hd) return CCS_ERR_INVALID;
   if (nsteps < 0) return fail(hd, CCS_ERR_INVALID, "ccs_step: nsteps < 0");
   if (!hd->mesh_set || !hd->state_set) return fail(hd, CCS_ERR_ORDER, "ccs_step before ccs_set_mesh/ccs_set_state");
+  if (hd->cfg.spatial_op == CCS_OP_WENO_EXT && !hd->ext_set)
+    return fail(hd, CCS_ERR_ORDER, "ccs_step: CCS_OP_WENO_EXT needs ccs_set_mesh_ext first");
   HIPCHK(hd, hipSetDevice(hd->cfg.device));
   int k = 0;
   if (nsteps > 0 && hd->explicit_valid) {
@@ -375,11 +425,14 @@ extern "C" int ccs_get_state(ccs_handle* hd, double* phiG, double* u, double* v,
 extern "C" int ccs_tendency(ccs_handle* hd, double* tp, double* tu, double* tv) {
   if (!hd) return CCS_ERR_INVALID;
   if (!hd->mesh_set || !hd->state_set) return fail(hd, CCS_ERR_ORDER, "ccs_tendency before set_mesh/set_state");
+  if (hd->cfg.spatial_op == CCS_OP_WENO_EXT && !hd->ext_set)
+    return fail(hd, CCS_ERR_ORDER, "ccs_tendency: CCS_OP_WENO_EXT needs ccs_set_mesh_ext first");
   HIPCHK(hd, hipSetDevice(hd->cfg.device));
   if (hd->cfg.precision != CCS_FP64_PARITY)
     return split_tendency(hd->split, hd->stream, hd->explicit_valid, tp, tu, tv, &hd->err);
   StageArgsF64 a{};
   a.n = hd->n; a.h = hd->h; a.dx = hd->cfg.dx; a.dt = hd->cfg.dt;
+  op_args(hd, a);
   a.phi0 = hd->phi0; a.uc0 = hd->uc0; a.vc0 = hd->vc0;
   a.in = Prog<const double>{hd->buf[0].phiG, hd->buf[0].u, hd->buf[0].v};
   a.base = a.in;
@@ -424,6 +477,8 @@ extern "C" int ccs_time_steps(ccs_handle* hd, int nsteps, double* total_ms, doub
   if (!hd) return CCS_ERR_INVALID;
   if (nsteps <= 0) return fail(hd, CCS_ERR_INVALID, "ccs_time_steps: nsteps must be positive");
   if (!hd->mesh_set || !hd->state_set) return fail(hd, CCS_ERR_ORDER, "ccs_time_steps before set_mesh/set_state");
+  if (hd->cfg.spatial_op == CCS_OP_WENO_EXT && !hd->ext_set)
+    return fail(hd, CCS_ERR_ORDER, "ccs_time_steps: CCS_OP_WENO_EXT needs ccs_set_mesh_ext first");
   HIPCHK(hd, hipSetDevice(hd->cfg.device));
   if (hd->explicit_valid) {  // the first step reads the uploaded companions: keep it out of the timing
     const int rc = ccs_step(hd, 1);

@@ -62,8 +62,16 @@ struct StageArgsF64 {
   double* tend[3];          // kTendOnly / kRK4a output (interior layout n*n*6)
   const double* tacc[9];    // kRK4f: L1, L2, L3 (phiG,u,v each)
   int n, h;
+  int op;                   // enum ccs_spatial_op
   double dx, dt, inc;
-  double rdx, r60dx;        // RN(1/dx), RN(1/(60 dx)); set by launch_stage_f64
+  // set by launch_stage_f64: RN(1/dx), RN(1/(60 dx)), 2 dx, RN(1/(2 dx)),
+  // RN(12 dx), RN(1/RN(12 dx))
+  double rdx, r60dx, dx2, rdx2, dx12, rdx12;
+  // CCS_OP_WENO_EXT face speeds: matrixIG_ext(1,1,2i+1,j) at x-face (i+1/2, j)
+  // for i = 0..n, j = 1..n: [i + (n+1)(j-1 + n p)]; matrixIG_ext(2,2,i,2j+1) at
+  // y-face (i, j+1/2) for i = 1..n, j = 0..n: [i-1 + n(j + (n+1) p)]
+  const double* igx;
+  const double* igy;
 };
 
 void launch_stage_f64(hipStream_t s, int kind, bool explicit_derived, const StageArgsF64& a,

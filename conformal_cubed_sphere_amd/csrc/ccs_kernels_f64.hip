@@ -93,14 +93,19 @@ __device__ __forceinline__ double d6(double m3, double m2, double m1, double p1,
   return div_known(-m3 + 9. * m2 - 45. * m1 + 45. * p1 - 9. * p2 + p3, 60. * dh, r60dh);
 }
 
-template <int KIND, bool EXPL>
+// Spatial operator variants (enum ccs_spatial_op): 0 WENO-Z, 1/2/3 centred
+// 2nd/4th/6th order, 4 WENO-Z with the extended-lattice metric and upwind
+// face winds in the vorticity.
+template <int KIND, bool EXPL, int OP>
 __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, HaloScatter hs) {
+  constexpr bool kWeno = (OP == 0 || OP == 4);
   // LDS, sized so that four workgroups fit a CU.  Fields read along both
   // directions cover the extended tile (row-major [lj][li], EX x EY); fields
   // read only along x cover the tile's rows (XB: [lj-3][li], EX x TY), those
   // read only along y the tile's columns (YB: [lj][li-3], TX x EY).  The face
   // values are written after a barrier into the storage of sFx/sFy/sE, which
-  // phase 3 no longer reads.
+  // phase 3 no longer reads.  OP 4 takes its face speeds from a table, so the
+  // storage of G^-1_11 / G^-1_22 holds its upwind face winds instead.
   constexpr int NXB = EX * TY, NYB = TX * EY;
   constexpr int NFX = (TX + 1) * TY;
   constexpr int NFY = TX * (TY + 1);
@@ -153,6 +158,7 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
     const double phiG = a.in.phiG[g], u = a.in.u[g], v = a.in.v[g];
     const double sg = st.sqrtG[g];
     const double g11 = st.IG11[g], g22 = st.IG22[g];
+    // (OP 4 reads G^-1 only for the contravariant winds; OP 0 also for the face speeds)
     double phi, uc, vc;
     if (EXPL) {
       phi = a.phi0[g]; uc = a.uc0[g]; vc = a.vc0[g];
@@ -172,14 +178,14 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
       const int lx = xb(lj, li);
       sFx[lx] = phiG * uc;
       sUC[lx] = uc;
-      sG11[lx] = g11;
+      if (OP == 0) sG11[lx] = g11;
       sV[lx] = v;
     }
     if (li >= 3 && li < TX + 3) {  // tile columns: y-direction fields
       const int ly = yb(lj, li);
       sFy[ly] = phiG * vc;
       sVC[ly] = vc;
-      sG22[ly] = g22;
+      if (OP == 0) sG22[ly] = g22;
       sU[ly] = u;
     }
   }
@@ -190,24 +196,27 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
   // the same code on strided LDS views, so no wave executes both variants.
   // Faces [2 NT, NFX + NFY) fill only part of a third pass; which wave takes
   // them rotates with the workgroup so that the SIMDs share the extra work.
-  constexpr int NFACE_IT = (NFX + NFY + NT - 1) / NT;
-  double fF[NFACE_IT], fE[NFACE_IT];
+  constexpr int NFACE_IT = kWeno ? (NFX + NFY + NT - 1) / NT : 0;
+  double fF[NFACE_IT + 1], fE[NFACE_IT + 1], fW[NFACE_IT + 1];
 #pragma unroll
   for (int it = 0; it < NFACE_IT; ++it) {
     const int e = it * NT + ((it == NFACE_IT - 1) ? ((tid + 64 * static_cast<int>(b & 3u)) & (NT - 1)) : tid);
     fF[it] = 0.;
     fE[it] = 0.;
+    fW[it] = 0.;
     if (e >= NFX + NFY) continue;
     // band offset/stride (fields read along one direction) and full-tile
     // offset/stride of the face's left (lower) cell
     int ob, sb, of, sf;
     bool ok;
+    long fq;  // OP 4: index of the face in its speed table
     if (e < NFX) {
       const int fi = e % (TX + 1), ty = e / (TX + 1);
       const int ig = i0 + fi - 1, j = j0 + ty;
       ok = j < n && ig >= -1 && ig <= n - 1;
       ob = ty * EX + fi + 2; sb = 1;           // x band: row ty of the tile
       of = (ty + 3) * EX + fi + 2; sf = 1;
+      fq = (ig + 1) + static_cast<long>(n + 1) * (j + static_cast<long>(n) * p);
     } else {
       const int e2 = e - NFX;
       const int tx = e2 % TX, fj = e2 / TX;
@@ -215,6 +224,7 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
       ok = i < n && jg >= -1 && jg <= n - 1;
       ob = (fj + 2) * TX + tx; sb = TX;        // y band: column tx of the tile
       of = (fj + 2) * EX + tx + 3; sf = EX;
+      fq = i + static_cast<long>(n) * ((jg + 1) + static_cast<long>(n + 1) * p);
     }
     if (!ok) continue;
     const double* F = (e < NFX ? sFx : sFy) + ob;
@@ -231,25 +241,39 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
     const double Pn = weno5(Q[3 * sf], Q[2 * sf], Q[sf], Q[0], Q[-sf]);
     const double uB = 0.5 * (U[sb] + U[0]);
     const double pB = 0.5 * (Pt[sf] + Pt[0]);
-    const double gq = 0.5 * (G[sb] + G[0]) * pB;
+    // OP 4: matrixIG_ext at lattice point (2i+1, j) / (i, 2j+1)
+    // (spatial_operators_mod_WENO.F90:111-112,134-135)
+    const double gq = (OP == 4) ? (e < NFX ? a.igx : a.igy)[fq] * pB : 0.5 * (G[sb] + G[0]) * pB;
     const double e1 = uB - sqrt(gq), e2 = uB + sqrt(gq);
     const double lam = fabs(e1) > fabs(e2) ? fabs(e1) : fabs(e2);
     fF[it] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
     fE[it] = 0.5 * (Ep + En - lam * (0. - 0.));
-  }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < NFACE_IT; ++it) {
-    const int e = it * NT + ((it == NFACE_IT - 1) ? ((tid + 64 * static_cast<int>(b & 3u)) & (NT - 1)) : tid);
-    if (e < NFX) {
-      sFxF[e] = fF[it];  // e = ty * (TX + 1) + fi
-      sExF[e] = fE[it];
-    } else if (e < NFX + NFY) {
-      sFyF[e - NFX] = fF[it];  // fj * TX + tx
-      sEyF[e - NFX] = fE[it];
+    if (OP == 4) {
+      // upwind face value of the tangential covariant wind: v at x faces, u at
+      // y faces (spatial_operators_mod_WENO.F90:88-89,98-99,118,141)
+      const double* W = (e < NFX ? sV : sU) + ob;
+      const double Wp = weno5(W[-2 * sb], W[-sb], W[0], W[sb], W[2 * sb]);
+      const double Wn = weno5(W[3 * sb], W[2 * sb], W[sb], W[0], W[-sb]);
+      fW[it] = 0.5 * (Wp + Wn - copysign(1., uB) * (Wn - Wp));
     }
   }
-  __syncthreads();
+  if (kWeno) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NFACE_IT; ++it) {
+      const int e = it * NT + ((it == NFACE_IT - 1) ? ((tid + 64 * static_cast<int>(b & 3u)) & (NT - 1)) : tid);
+      if (e < NFX) {
+        sFxF[e] = fF[it];  // e = ty * (TX + 1) + fi
+        sExF[e] = fE[it];
+        if (OP == 4) sG11[e] = fW[it];
+      } else if (e < NFX + NFY) {
+        sFyF[e - NFX] = fF[it];  // fj * TX + tx
+        sEyF[e - NFX] = fE[it];
+        if (OP == 4) sG22[e - NFX] = fW[it];
+      }
+    }
+    __syncthreads();
+  }
 
   // ---- phase 3: cells
   const int tx = tid % TX, ty = tid / TX;
@@ -257,15 +281,42 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
   if (i >= n || j >= n) return;
   const double dx = a.dx;
   const double rdx = a.rdx, r60 = a.r60dx;
-  const double dfluxdx = div_known(sFxF[ty * (TX + 1) + tx + 1] - sFxF[ty * (TX + 1) + tx], dx, rdx);
-  const double dEdx = div_known(sExF[ty * (TX + 1) + tx + 1] - sExF[ty * (TX + 1) + tx], dx, rdx);
   const int lxc = ty * EX + (tx + 3);   // x band, this cell
   const int lyc = (ty + 3) * TX + tx;   // y band, this cell
-  const double dvdx = d6(sV[lxc - 3], sV[lxc - 2], sV[lxc - 1], sV[lxc + 1], sV[lxc + 2], sV[lxc + 3], dx, r60);
-  const double dfluxdy = div_known(sFyF[(ty + 1) * TX + tx] - sFyF[ty * TX + tx], dx, rdx);
-  const double dEdy = div_known(sEyF[(ty + 1) * TX + tx] - sEyF[ty * TX + tx], dx, rdx);
-  const double dudy = d6(sU[lyc - 3 * TX], sU[lyc - 2 * TX], sU[lyc - TX], sU[lyc + TX], sU[lyc + 2 * TX],
-                         sU[lyc + 3 * TX], dx, r60);
+  const int lc = (ty + 3) * EX + (tx + 3);
+  double dfluxdx, dEdx, dvdx, dfluxdy, dEdy, dudy;
+  if (kWeno) {
+    const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
+    dfluxdx = div_known(sFxF[fx + 1] - sFxF[fx], dx, rdx);
+    dEdx = div_known(sExF[fx + 1] - sExF[fx], dx, rdx);
+    dfluxdy = div_known(sFyF[fy + TX] - sFyF[fy], dx, rdx);
+    dEdy = div_known(sEyF[fy + TX] - sEyF[fy], dx, rdx);
+    if (OP == 4) {  // spatial_operators_mod_WENO.F90:160,163
+      dvdx = div_known(sG11[fx + 1] - sG11[fx], dx, rdx);
+      dudy = div_known(sG22[fy + TX] - sG22[fy], dx, rdx);
+    } else {
+      dvdx = d6(sV[lxc - 3], sV[lxc - 2], sV[lxc - 1], sV[lxc + 1], sV[lxc + 2], sV[lxc + 3], dx, r60);
+      dudy = d6(sU[lyc - 3 * TX], sU[lyc - 2 * TX], sU[lyc - TX], sU[lyc + TX], sU[lyc + 2 * TX],
+                sU[lyc + 3 * TX], dx, r60);
+    }
+  } else {
+    // centred differences of the cell values (x band: stride 1, y band:
+    // stride TX, full tile: stride EX)
+    auto cd = [&](const double* f, int l, int s) {
+      if (OP == 1)  // spatial_operators_mod_2nd.F90:54-59
+        return div_known(f[l + s] - f[l - s], a.dx2, a.rdx2);
+      if (OP == 2)  // spatial_operators_mod_4th.F90:58-63
+        return div_known(f[l - 2 * s] - 8. * f[l - s] + 8. * f[l + s] - f[l + 2 * s], a.dx12, a.rdx12);
+      // spatial_operators_mod_6th.F90:64-69
+      return d6(f[l - 3 * s], f[l - 2 * s], f[l - s], f[l + s], f[l + 2 * s], f[l + 3 * s], dx, r60);
+    };
+    dfluxdx = cd(sFx, lxc, 1);
+    dEdx = cd(sE, lc, 1);
+    dvdx = cd(sV, lxc, 1);
+    dfluxdy = cd(sFy, lyc, TX);
+    dEdy = cd(sE, lc, EX);
+    dudy = cd(sU, lyc, TX);
+  }
   const long g = gidx(i, j, p, h, P);
   const double vort = dvdx - dudy + st.sqrtG[g] * st.f[g];
   const double tp = -(dfluxdx + dfluxdy);
@@ -453,12 +504,26 @@ void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_
   StageArgsF64 a = a_in;
   a.rdx = 1.0 / a.dx;            // correctly rounded reciprocals for div_known
   a.r60dx = 1.0 / (60.0 * a.dx);
+  a.dx2 = 2.0 * a.dx;
+  a.rdx2 = 1.0 / a.dx2;
+  a.dx12 = 12.0 * a.dx;
+  a.rdx12 = 1.0 / a.dx12;
   const dim3 grid(static_cast<unsigned>(((a.n + TX - 1) / TX) * ((a.n + TY - 1) / TY) * 6));
   const dim3 block(NT);
-#define CCS_LAUNCH(K)                                                                   \
-  do {                                                                                 \
-    if (expl) hipLaunchKernelGGL((stage_f64<K, true>), grid, block, 0, s, a, st, hs);  \
-    else hipLaunchKernelGGL((stage_f64<K, false>), grid, block, 0, s, a, st, hs);      \
+#define CCS_LAUNCH_OP(K, O)                                                                \
+  do {                                                                                    \
+    if (expl) hipLaunchKernelGGL((stage_f64<K, true, O>), grid, block, 0, s, a, st, hs);  \
+    else hipLaunchKernelGGL((stage_f64<K, false, O>), grid, block, 0, s, a, st, hs);      \
+  } while (0)
+#define CCS_LAUNCH(K)                      \
+  do {                                     \
+    switch (a.op) {                        \
+      case 1: CCS_LAUNCH_OP(K, 1); break;  \
+      case 2: CCS_LAUNCH_OP(K, 2); break;  \
+      case 3: CCS_LAUNCH_OP(K, 3); break;  \
+      case 4: CCS_LAUNCH_OP(K, 4); break;  \
+      default: CCS_LAUNCH_OP(K, 0); break; \
+    }                                      \
   } while (0)
   switch (kind) {
     case kTendOnly: CCS_LAUNCH(kTendOnly); break;
@@ -469,6 +534,7 @@ void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_
     default: CCS_LAUNCH(kRK4f); break;
   }
 #undef CCS_LAUNCH
+#undef CCS_LAUNCH_OP
 }
 
 void launch_derived_f64(hipStream_t s, const Prog<const double>& q, const StaticsF64& st, int n, int h,

@@ -534,3 +534,49 @@ extern "C" int ccs_mesh_metrics(const double* lon_deg, const double* lat_deg, co
       }
   return status;
 }
+
+// Inverse metric on the extended lattice, io_mod.F90:223-250: for every
+// extended point except the panel's four corner points (left unset there, so
+// zero here), G_ext = J_ext^T J_ext with J_ext = jab_ext * radius and
+// matrixIG_ext = BRINV(G_ext).
+extern "C" int ccs_mesh_ext_metric(int n, double radius, const double* jab_ext, double* ig_ext) {
+  if (n < 4 || (n % 2) != 0 || !ig_ext || !(radius > 0.0)) return CCS_ERR_INVALID;
+  const int m = 2 * n + 1;
+  const std::size_t np = static_cast<std::size_t>(m) * m;
+  std::vector<double> gen;
+  if (!jab_ext) {
+    std::vector<double> coord(m);
+    for (int k = 0; k < m; ++k) coord[k] = -1.0 + static_cast<double>(k) / n;
+    std::vector<std::complex<double>> w, dwdz;
+    map_points(coord, w, dwdz);
+    gen.resize(36 * np);
+    for (int p = 0; p < 6; ++p)
+      for (std::size_t c = 0; c < np; ++c) {
+        double g[3];
+        panel_point(p, w[c], dwdz[c], g, &gen[6 * (c + np * p)]);
+      }
+    jab_ext = gen.data();
+  }
+  int status = CCS_OK;
+  for (int p = 0; p < 6; ++p)
+    for (int j = 1; j <= m; ++j)
+      for (int i = 1; i <= m; ++i) {
+        const std::size_t q = static_cast<std::size_t>(i - 1) + static_cast<std::size_t>(m) *
+                              (static_cast<std::size_t>(j - 1) + static_cast<std::size_t>(m) * p);
+        double* IG = ig_ext + 4 * q;
+        IG[0] = IG[1] = IG[2] = IG[3] = 0.0;
+        if ((i == 1 || i == m) && (j == 1 || j == m)) continue;
+        double J[6];
+        for (int a = 0; a < 6; ++a) J[a] = jab_ext[6 * q + a] * radius;  // io_mod.F90:115
+        double G[4];
+        for (int c = 0; c < 2; ++c)
+          for (int r = 0; r < 2; ++r) {
+            double acc = J[0 + 3 * r] * J[0 + 3 * c];
+            acc = acc + J[1 + 3 * r] * J[1 + 3 * c];
+            acc = acc + J[2 + 3 * r] * J[2 + 3 * c];
+            G[r + 2 * c] = acc;
+          }
+        if (!brinv2(G, IG)) status = CCS_ERR_SINGULAR;
+      }
+  return status;
+}

@@ -315,7 +315,8 @@ extern "C" int ccs_mesh_file_write(const char* path, int n) {
   return ok ? CCS_OK : CCS_ERR_INVALID;
 }
 
-extern "C" int ccs_mesh_file_read(const char* path, int* n_out, double* lon_deg, double* lat_deg, double* jab) {
+namespace {
+int read_mesh(const char* path, int* n_out, double* lon_deg, double* lat_deg, double* jab, double* jab_ext) {
   if (!path || !n_out) return CCS_ERR_INVALID;
   FILE* f = std::fopen(path, "rb");
   if (!f) return CCS_ERR_INVALID;
@@ -402,7 +403,18 @@ extern "C" int ccs_mesh_file_read(const char* path, int* n_out, double* lon_deg,
         if (jab)
           for (int k = 0; k < 6; ++k) jab[6 * c + k] = r.be64(vj->begin + 8 * (6 * e + k));
       }
+  if (jab_ext)  // the whole extended lattice, (3,2,lon,lat,nPatch) as stored
+    for (std::size_t k = 0; k < 36 * np; ++k) jab_ext[k] = r.be64(vj->begin + 8 * k);
   return CCS_OK;
+}
+}  // namespace
+
+extern "C" int ccs_mesh_file_read(const char* path, int* n_out, double* lon_deg, double* lat_deg, double* jab) {
+  return read_mesh(path, n_out, lon_deg, lat_deg, jab, nullptr);
+}
+
+extern "C" int ccs_mesh_file_read_ext(const char* path, int* n_out, double* jab_ext) {
+  return read_mesh(path, n_out, nullptr, nullptr, nullptr, jab_ext);
 }
 
 // ------------------------------------------------------------ history file

@@ -13,6 +13,8 @@ module ccs_gpu_mod
   integer(c_int), parameter :: CCS_FP64_PARITY = 0, CCS_FP32_SPLIT = 1, CCS_FP64_FAST = 2
   integer(c_int), parameter :: CCS_RK3_TVD = 0, CCS_RK4 = 1
   integer(c_int), parameter :: CCS_EFLUX_NONE = 0
+  integer(c_int), parameter :: CCS_OP_WENO = 0, CCS_OP_CENTRED2 = 1, CCS_OP_CENTRED4 = 2, &
+                               CCS_OP_CENTRED6 = 3, CCS_OP_WENO_EXT = 4
 
   type, bind(C) :: ccs_metrics
     integer(c_int) :: n, halo
@@ -30,7 +32,7 @@ module ccs_gpu_mod
   end type ccs_case_params
 
   type, bind(C) :: ccs_config
-    integer(c_int) :: n, halo, precision, scheme, eflux, device, use_graph, pad_
+    integer(c_int) :: n, halo, precision, scheme, eflux, device, use_graph, spatial_op
     real(c_double) :: dt, dx, radius, gravity, omega
   end type ccs_config
 
@@ -51,6 +53,27 @@ module ccs_gpu_mod
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int) :: n
       type(c_ptr), value :: lon_deg, lat_deg, jab
+    end function
+
+    integer(c_int) function ccs_mesh_file_read_ext(path, n, jab_ext) bind(C, name="ccs_mesh_file_read_ext")
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: n
+      type(c_ptr), value :: jab_ext
+    end function
+
+    integer(c_int) function ccs_mesh_ext_metric(n, radius, jab_ext, ig_ext) bind(C, name="ccs_mesh_ext_metric")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: n
+      real(c_double), value :: radius
+      type(c_ptr), value :: jab_ext
+      real(c_double) :: ig_ext(*)
+    end function
+
+    integer(c_int) function ccs_set_mesh_ext(h, ig_ext) bind(C, name="ccs_set_mesh_ext")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ig_ext(*)
     end function
 
     integer(c_int) function ccs_history_create(path, d, lon, lat, areaCell, phis, h) bind(C, name="ccs_history_create")

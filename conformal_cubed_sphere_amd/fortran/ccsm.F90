@@ -8,6 +8,13 @@
 ! (ccsm.F90:50-67) and the same "output index/total, MCR, ECR" line.  The
 ! time step itself runs on the GPU through libccs.so (ccs_gpu_mod).
 !
+! &dynamic_opt reconstruct_scheme (declared but unused by the reference, which
+! swaps spatial_operators_mod sources in its build instead) selects the
+! operator: 'WENO' (spatial_operators_mod.F90, the default), '2nd', '4th',
+! '6th' (spatial_operators_mod_2nd/4th/6th.F90) or 'WENO_ext'
+! (spatial_operators_mod_WENO.F90, with mesh%matrixIG_ext from the mesh file's
+! jab or the generated map).  The variants run with precision = 'fp64'.
+!
 ! Differences, all in the optional group &gpu_opt that follows &dynamic_opt:
 !   nx         cells per panel edge.  When &domain mesh_file names an existing
 !              file of the generator's schema (classic or 64-bit-offset
@@ -87,7 +94,11 @@ contains
     if (trim(adjustl(integral_scheme)) /= 'RK3_TVD' .and. trim(adjustl(integral_scheme)) /= 'RK4') then
       error stop 'Unknown integral scheme, please select from RK3_TVD or RK4 ...'
     end if
-    if (trim(adjustl(reconstruct_scheme)) /= 'WENO') error stop 'Only reconstruct_scheme = WENO is implemented'
+    select case (trim(adjustl(reconstruct_scheme)))
+    case ('WENO', '2nd', '4th', '6th', 'WENO_ext')
+    case default
+      error stop 'reconstruct_scheme must be WENO, 2nd, 4th, 6th or WENO_ext'
+    end select
     if (xhalo /= yhalo) error stop 'xhalo and yhalo must be equal'
     if (xhalo < 3) error stop 'xhalo >= 3 is required (WENO-5 faces and the 6th-order vorticity)'
     inquire(file=trim(mesh_file), exist=mesh_from_file)
@@ -111,6 +122,7 @@ program ccsm
   real(8), allocatable, target :: lon(:), lat(:), sinlon(:), coslon(:), sinlat(:), coslat(:)
   real(8), allocatable, target :: sqrtG(:), f(:), area(:), G(:), IG(:), A(:), IA(:)
   real(8), allocatable :: phiG(:), u(:), v(:), phi(:), uc(:), vc(:), zonal(:), merid(:), phis(:)
+  real(8), allocatable, target :: jab_ext(:), ig_ext(:)
   type(ccs_metrics) :: m
   type(ccs_case_params) :: cp
   type(ccs_config) :: cfg
@@ -183,7 +195,8 @@ program ccsm
     print *, 'dt from CFL ', cfl, ': ', dt, ' s, ', nsteps, ' steps'
   end if
   print *, ''
-  print *, 'Temporal integration scheme is ' // trim(adjustl(integral_scheme)), ', precision ', trim(precision)
+  print *, 'Temporal integration scheme is ' // trim(adjustl(integral_scheme)), ', precision ', trim(precision), &
+           ', reconstruct_scheme ', trim(adjustl(reconstruct_scheme))
   print *, ''
 
   ! ---- GPU state (initStat/initTend)
@@ -203,7 +216,18 @@ program ccsm
   cfg%eflux = CCS_EFLUX_NONE
   cfg%device = device
   cfg%use_graph = use_graph
-  cfg%pad_ = 0
+  select case (trim(adjustl(reconstruct_scheme)))
+  case ('2nd')
+    cfg%spatial_op = CCS_OP_CENTRED2
+  case ('4th')
+    cfg%spatial_op = CCS_OP_CENTRED4
+  case ('6th')
+    cfg%spatial_op = CCS_OP_CENTRED6
+  case ('WENO_ext')
+    cfg%spatial_op = CCS_OP_WENO_EXT
+  case default
+    cfg%spatial_op = CCS_OP_WENO
+  end select
   cfg%dt = dt
   cfg%dx = m%dx
   cfg%radius = radius
@@ -211,6 +235,20 @@ program ccsm
   cfg%omega = omega
   call ccs_check(ccs_create(cfg, hnd), 'ccs_create')
   call ccs_check(ccs_set_mesh(hnd, sqrtG, G, IG, A, IA, f, phis), 'ccs_set_mesh', hnd)
+  if (cfg%spatial_op == CCS_OP_WENO_EXT) then
+    ! mesh%matrixIG_ext (io_mod.F90:223-250) from the mesh file's jab when
+    ! the grid came from a file, else from the generated conformal map
+    allocate(ig_ext(4 * (2 * n + 1)**2 * 6))
+    if (mesh_from_file) then
+      allocate(jab_ext(36 * (2 * n + 1)**2))
+      call ccs_check(ccs_mesh_file_read_ext(trim(mesh_file) // c_null_char, k, c_loc(jab_ext)), &
+                     'ccs_mesh_file_read_ext ' // trim(mesh_file))
+      call ccs_check(ccs_mesh_ext_metric(n, radius, c_loc(jab_ext), ig_ext), 'ccs_mesh_ext_metric')
+    else
+      call ccs_check(ccs_mesh_ext_metric(n, radius, c_null_ptr, ig_ext), 'ccs_mesh_ext_metric')
+    end if
+    call ccs_check(ccs_set_mesh_ext(hnd, ig_ext), 'ccs_set_mesh_ext', hnd)
+  end if
   call ccs_check(ccs_set_state(hnd, phiG, u, v, phi, uc, vc), 'ccs_set_state', hnd)
 
   output_idx = 1

@@ -63,6 +63,29 @@ enum ccs_scheme { CCS_RK3_TVD = 0, CCS_RK4 = 1 };
  * on fresh stack pages: CCS_EFLUX_NONE reproduces that reading. */
 enum ccs_eflux { CCS_EFLUX_NONE = 0 };
 
+/* Spatial operator: which of the reference's interchangeable
+ * spatial_operators_mod sources the build compiles in (the namelist's
+ * reconstruct_scheme, parameters_mod.F90:30, selects none of them; the module
+ * file is swapped in the build).  Only CCS_FP64_PARITY runs the variants.
+ *  CCS_OP_WENO      spatial_operators_mod.F90: WENO-Z + Rusanov, face speed
+ *                   from the mean of the two cells' G^-1, 6th-order vorticity
+ *  CCS_OP_CENTRED2  spatial_operators_mod_2nd.F90: centred 2nd-order
+ *                   differences of the cell fluxes, E and winds
+ *  CCS_OP_CENTRED4  spatial_operators_mod_4th.F90: centred 4th order
+ *  CCS_OP_CENTRED6  spatial_operators_mod_6th.F90: centred 6th order
+ *  CCS_OP_WENO_EXT  spatial_operators_mod_WENO.F90: WENO-Z + Rusanov with the
+ *                   face speed from mesh%matrixIG_ext (read at lattice point
+ *                   (2i+1, j) / (i, 2j+1) as written there, :111-112,134-135)
+ *                   and the vorticity from upwind WENO face winds (:118,141,
+ *                   160,163); needs ccs_set_mesh_ext. */
+enum ccs_spatial_op {
+  CCS_OP_WENO = 0,
+  CCS_OP_CENTRED2 = 1,
+  CCS_OP_CENTRED4 = 2,
+  CCS_OP_CENTRED6 = 3,
+  CCS_OP_WENO_EXT = 4
+};
+
 /* ---------------------------------------------------------- mesh producer */
 /* Conformal cubed-sphere cell-centre geometry on the reference panel layout
  * (mesh_generation/src/cuco.f90:100-105 frames, mesh_mod.f90:150-166 output):
@@ -92,6 +115,15 @@ typedef struct ccs_metrics {
 
 int ccs_mesh_metrics(const double* lon_deg, const double* lat_deg,
                      const double* jab, ccs_metrics* m);
+
+/* mesh%matrixIG_ext as io_mod.F90:223-250 computes it: BRINV(J^T J) with
+ * J = jab_ext * radius at every extended-lattice point except each panel's
+ * four corner points, which the reference leaves unset (zero here).
+ * jab_ext: the mesh file's jab, (3,2,2n+1,2n+1,6) per radian
+ * (ccs_mesh_file_read_ext), or NULL for this library's own conformal map.
+ * matrixIG_ext: caller-allocated 4*(2n+1)^2*6. */
+int ccs_mesh_ext_metric(int n, double radius, const double* jab_ext,
+                        double* matrixIG_ext);
 
 /* ---------------------------------------------------------- initial state */
 /* Initial conditions: Williamson cases 2, 5, 6 of test_case_mod.F90:61-204
@@ -133,7 +165,7 @@ typedef struct ccs_config {
   int eflux;             /* enum ccs_eflux                                    */
   int device;            /* HIP device ordinal                                */
   int use_graph;         /* replay each step from a captured HIP graph        */
-  int pad_;
+  int spatial_op;        /* enum ccs_spatial_op (0: the reference default)    */
   double dt, dx;         /* seconds; radians (ccs_metrics.dx)                 */
   double radius, gravity, omega;
 } ccs_config;
@@ -147,6 +179,11 @@ int ccs_create(const ccs_config* cfg, ccs_handle** out);
 int ccs_set_mesh(ccs_handle* h, const double* sqrtG, const double* G,
                  const double* IG, const double* A, const double* IA,
                  const double* f, const double* phis);
+
+/* Upload mesh%matrixIG_ext (mesh_mod.F90:45, io_mod.F90:223-250): the inverse
+ * metric on the extended lattice, (2,2,2n+1,2n+1,6) column-major.  Needed by
+ * CCS_OP_WENO_EXT only (rejected otherwise). */
+int ccs_set_mesh_ext(ccs_handle* h, const double* matrixIG_ext);
 
 /* Upload the prognostic state stat(old) (stat_mod.F90:7-17): padded phiG and
  * covariant u, v.  phi, uc, vc are the diagnostic companions the reference
@@ -206,6 +243,10 @@ int ccs_mesh_file_write(const char* path, int n);
  * ccs_mesh_geometry's outputs.  Output arrays may be NULL (n only). */
 int ccs_mesh_file_read(const char* path, int* n, double* lon_deg,
                        double* lat_deg, double* jab);
+/* The whole jab variable of such a file (io_mod.F90:107, mesh%jab_ext):
+ * (3,2,2n+1,2n+1,6) per radian, for ccs_mesh_ext_metric.  jab_ext may be
+ * NULL (n only). */
+int ccs_mesh_file_read_ext(const char* path, int* n, double* jab_ext);
 
 /* History file ccsm_output.nc of io_mod.F90:309-541 (64-bit-offset netCDF,
  * as there): lon/lat (degrees, corners _FillValue), areaCell and phis on the

@@ -451,6 +451,63 @@ void oracle_spatial_operator(const oracle_mesh* m, const oracle_stat* s, double*
     Fy[d] = s->phiG[d] * s->vc[d];
   }
 
+  if (m->op >= 1 && m->op <= 3) {
+    /* centred differences of the cell values, spatial_operators_mod_2nd.F90:46-67,
+     * _4th.F90:46-71, _6th.F90:46-77 (same pointwise terms as above) */
+    const int op = m->op;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 1; p <= 6; ++p)
+      for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= n; ++i) {
+          size_t d = PIDX(n, h, i, j, p);
+          double fx[7], ex[7], vx[7], fy[7], ey[7], uy[7];
+          for (int a = 0; a < 7; ++a) {
+            if (a == 3 || (op == 1 && (a == 0 || a == 1 || a == 5 || a == 6)) || (op == 2 && (a == 0 || a == 6)))
+              continue;
+            size_t dxi = PIDX(n, h, i - 3 + a, j, p), dyj = PIDX(n, h, i, j - 3 + a, p);
+            fx[a] = Fx[dxi]; ex[a] = E[dxi]; vx[a] = s->v[dxi];
+            fy[a] = Fy[dyj]; ey[a] = E[dyj]; uy[a] = s->u[dyj];
+          }
+          double dfluxdx, dEdx, dvdx, dfluxdy, dEdy, dudy;
+          if (op == 1) {
+            dfluxdx = (fx[4] - fx[2]) / (2. * dx);
+            dEdx = (ex[4] - ex[2]) / (2. * dx);
+            dvdx = (vx[4] - vx[2]) / (2. * dx);
+            dfluxdy = (fy[4] - fy[2]) / (2. * dy);
+            dEdy = (ey[4] - ey[2]) / (2. * dy);
+            dudy = (uy[4] - uy[2]) / (2. * dy);
+          } else if (op == 2) {
+            dfluxdx = (fx[1] - 8. * fx[2] + 8. * fx[4] - fx[5]) / (12. * dx);
+            dEdx = (ex[1] - 8. * ex[2] + 8. * ex[4] - ex[5]) / (12. * dx);
+            dvdx = (vx[1] - 8. * vx[2] + 8. * vx[4] - vx[5]) / (12. * dx);
+            dfluxdy = (fy[1] - 8. * fy[2] + 8. * fy[4] - fy[5]) / (12. * dy);
+            dEdy = (ey[1] - 8. * ey[2] + 8. * ey[4] - ey[5]) / (12. * dy);
+            dudy = (uy[1] - 8. * uy[2] + 8. * uy[4] - uy[5]) / (12. * dy);
+          } else {
+            dfluxdx = oracle_d6(fx, dx);
+            dEdx = oracle_d6(ex, dx);
+            dvdx = oracle_d6(vx, dx);
+            dfluxdy = oracle_d6(fy, dy);
+            dEdy = oracle_d6(ey, dy);
+            dudy = oracle_d6(uy, dy);
+          }
+          double vort = dvdx - dudy + m->sqrtG[d] * m->f[d];
+          t_phiG[d] = -(dfluxdx + dfluxdy);
+          t_u[d] = -dEdx + vort * s->vc[d];
+          t_v[d] = -dEdy - vort * s->uc[d];
+        }
+    free(phit); free(phitG); free(E); free(Fx); free(Fy);
+    free(fxF); free(fxE); free(fyF); free(fyE);
+    return;
+  }
+  /* op 4: face-lattice G^-1 and upwind face winds (spatial_operators_mod_WENO.F90) */
+  const int ext = m->op == 4;
+  const size_t me = (size_t)(2 * n + 1);
+  /* mesh%matrixIG_ext(r,c,a,b,p), 1-based indices */
+#define IGX(r, c, a, b, p) m->IG_ext[((r)-1) + 2 * ((c)-1) + 4 * (((size_t)(a)-1) + me * (((size_t)(b)-1) + me * ((size_t)(p)-1)))]
+  double* fxV = ext ? malloc(nf * sizeof(double)) : NULL;
+  double* fyU = ext ? malloc(nf * sizeof(double)) : NULL;
+
   /* x faces, spatial_operators_mod.F90:80-124 */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int p = 1; p <= 6; ++p)
@@ -468,12 +525,24 @@ void oracle_spatial_operator(const oracle_mesh* m, const oracle_stat* s, double*
         size_t d0 = PIDX(n, h, i, j, p), d1 = PIDX(n, h, i + 1, j, p);
         double uBdy = 0.5 * (s->uc[d1] + s->uc[d0]);
         double phitBdy = 0.5 * (phit[d1] + phit[d0]);
-        double g = 0.5 * (m->IG[4 * d1 + 0] + m->IG[4 * d0 + 0]) * phitBdy;
+        /* op 4 reads matrixIG_ext(1,1,2i+1,j) -- row j, not 2j, as written at
+         * spatial_operators_mod_WENO.F90:111-112 */
+        double g = ext ? IGX(1, 1, 2 * i + 1, j, p) * phitBdy
+                       : 0.5 * (m->IG[4 * d1 + 0] + m->IG[4 * d0 + 0]) * phitBdy;
         double e1 = uBdy - sqrt(g), e2 = uBdy + sqrt(g);
         double lam = fabs(e1) > fabs(e2) ? fabs(e1) : fabs(e2);
         size_t f = (size_t)i + (size_t)(n + 1) * ((size_t)(j - 1) + (size_t)n * (size_t)(p - 1));
         fxF[f] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
         fxE[f] = 0.5 * (Ep + En - lam * (0. - 0.)); /* u_ext_n - u_ext_p: never written */
+        if (ext) { /* v_ext, spatial_operators_mod_WENO.F90:88-89,118 */
+          double qV[5], rV[5];
+          for (int a = 0; a < 5; ++a) {
+            qV[a] = s->v[PIDX(n, h, i - 2 + a, j, p)];
+            rV[a] = s->v[PIDX(n, h, i - 1 + a, j, p)];
+          }
+          double Vp = oracle_weno_limiter(qV, 1), Vn = oracle_weno_limiter(rV, -1);
+          fxV[f] = 0.5 * (Vp + Vn - copysign(1., uBdy) * (Vn - Vp));
+        }
       }
 
   /* y faces, spatial_operators_mod.F90:90-99,126-149 */
@@ -493,12 +562,24 @@ void oracle_spatial_operator(const oracle_mesh* m, const oracle_stat* s, double*
         size_t d0 = PIDX(n, h, i, j, p), d1 = PIDX(n, h, i, j + 1, p);
         double vBdy = 0.5 * (s->vc[d1] + s->vc[d0]);
         double phitBdy = 0.5 * (phit[d1] + phit[d0]);
-        double g = 0.5 * (m->IG[4 * d1 + 3] + m->IG[4 * d0 + 3]) * phitBdy;
+        /* op 4 reads matrixIG_ext(2,2,i,2j+1) -- column i, not 2i, as written at
+         * spatial_operators_mod_WENO.F90:134-135 */
+        double g = ext ? IGX(2, 2, i, 2 * j + 1, p) * phitBdy
+                       : 0.5 * (m->IG[4 * d1 + 3] + m->IG[4 * d0 + 3]) * phitBdy;
         double e1 = vBdy - sqrt(g), e2 = vBdy + sqrt(g);
         double lam = fabs(e1) > fabs(e2) ? fabs(e1) : fabs(e2);
         size_t f = (size_t)(i - 1) + (size_t)n * ((size_t)j + (size_t)(n + 1) * (size_t)(p - 1));
         fyF[f] = 0.5 * (Fp + Fn - lam * (Pn - Pp));
         fyE[f] = 0.5 * (Ep + En - lam * (0. - 0.)); /* v_ext_n - v_ext_p: never written */
+        if (ext) { /* u_ext, spatial_operators_mod_WENO.F90:98-99,141 */
+          double qU[5], rU[5];
+          for (int a = 0; a < 5; ++a) {
+            qU[a] = s->u[PIDX(n, h, i, j - 2 + a, p)];
+            rU[a] = s->u[PIDX(n, h, i, j - 1 + a, p)];
+          }
+          double Up = oracle_weno_limiter(qU, 1), Un = oracle_weno_limiter(rU, -1);
+          fyU[f] = 0.5 * (Up + Un - copysign(1., vBdy) * (Un - Up));
+        }
       }
 
   /* divergence, vorticity and tendencies, spatial_operators_mod.F90:151-179 */
@@ -518,17 +599,18 @@ void oracle_spatial_operator(const oracle_mesh* m, const oracle_stat* s, double*
         }
         double dfluxdx = (fxF[fe] - fxF[fw]) / dx;
         double dEdx = (fxE[fe] - fxE[fw]) / dx;
-        double dvdx = oracle_d6(vx, dx);
+        double dvdx = ext ? (fxV[fe] - fxV[fw]) / dx : oracle_d6(vx, dx);   /* _WENO.F90:160 */
         double dfluxdy = (fyF[fn] - fyF[fs]) / dy;
         double dEdy = (fyE[fn] - fyE[fs]) / dy;
-        double dudy = oracle_d6(uy, dy);
+        double dudy = ext ? (fyU[fn] - fyU[fs]) / dy : oracle_d6(uy, dy);   /* _WENO.F90:163 */
         double vort = dvdx - dudy + m->sqrtG[d] * m->f[d];
         t_phiG[d] = -(dfluxdx + dfluxdy);
         t_u[d] = -dEdx + vort * s->vc[d];
         t_v[d] = -dEdy - vort * s->uc[d];
       }
   free(phit); free(phitG); free(E); free(Fx); free(Fy);
-  free(fxF); free(fxE); free(fyF); free(fyE);
+  free(fxF); free(fxE); free(fyF); free(fyE); free(fxV); free(fyU);
+#undef IGX
 }
 
 /* --------------------------------------------------------- ghost update */

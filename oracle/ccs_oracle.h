@@ -15,10 +15,22 @@
 extern "C" {
 #endif
 
+/* Spatial operator: the module the reference build compiles in.
+ *   0  spatial_operators_mod.F90       WENO-Z, Rusanov speed from the mean of
+ *                                      the two cells' G^-1, 6th-order vorticity
+ *   1  spatial_operators_mod_2nd.F90   centred 2nd-order differences
+ *   2  spatial_operators_mod_4th.F90   centred 4th-order differences
+ *   3  spatial_operators_mod_6th.F90   centred 6th-order differences
+ *   4  spatial_operators_mod_WENO.F90  WENO-Z, Rusanov speed from the
+ *                                      extended-lattice G^-1, upwind WENO face
+ *                                      winds in the vorticity */
 typedef struct oracle_mesh {
   int n, h;
   double dx, dt;
   const double *sqrtG, *G, *IG, *A, *IA, *f, *phis; /* padded */
+  int op, pad_;
+  /* op 4: mesh%matrixIG_ext, (2,2,2n+1,2n+1,6) column-major (mesh_mod.F90:45) */
+  const double* IG_ext;
 } oracle_mesh;
 
 /* The nine fields of stat_field (stat_mod.F90:7-17), padded. */

@@ -102,6 +102,8 @@ contains
     merid = stat(k)%meridional_wind
   end subroutine ref_get_state
 
+#ifndef CCS_REF_CENTRED
+  ! the centred-difference operator modules have no WENO_limiter
   real(c_double) function ref_weno(q, dir) bind(C, name="ref_weno")
     real(c_double), intent(in) :: q(5)
     integer(c_int), value :: dir
@@ -109,6 +111,13 @@ contains
     call WENO_limiter(r, q, dir)
     ref_weno = r
   end function ref_weno
+#endif
+
+  ! mesh%matrixIG_ext (io_mod.F90:223-250), read by spatial_operators_mod_WENO.F90
+  subroutine ref_set_mesh_ext(IG_ext) bind(C, name="ref_set_mesh_ext")
+    real(c_double), intent(in) :: IG_ext(2, 2, ids:ide, jds:jde, 6)
+    mesh%matrixIG_ext = IG_ext
+  end subroutine ref_set_mesh_ext
 
   subroutine call_spatial_operator(k, t)
     integer, intent(in) :: k, t

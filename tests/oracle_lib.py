@@ -20,7 +20,12 @@ _D = ctypes.POINTER(ctypes.c_double)
 
 class OracleMesh(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("h", ctypes.c_int), ("dx", ctypes.c_double), ("dt", ctypes.c_double),
-                ("sqrtG", _D), ("G", _D), ("IG", _D), ("A", _D), ("IA", _D), ("f", _D), ("phis", _D)]
+                ("sqrtG", _D), ("G", _D), ("IG", _D), ("A", _D), ("IA", _D), ("f", _D), ("phis", _D),
+                ("op", ctypes.c_int), ("pad_", ctypes.c_int), ("IG_ext", _D)]
+
+
+# spatial operator variants (oracle_mesh.op)
+OPS = {"weno": 0, "centred2": 1, "centred4": 2, "centred6": 3, "weno_ext": 4}
 
 
 class OracleStat(ctypes.Structure):
@@ -114,11 +119,14 @@ def init_case(mesh, case_num=7, seed=1, n_bumps=8, mountain=False, gravity=9.806
 class Oracle:
     """A CPU run of the reference time step on a product Mesh and a State."""
 
-    def __init__(self, mesh, state, dt, threads=0):
+    def __init__(self, mesh, state, dt, threads=0, op="weno", ig_ext=None):
         self.mesh = mesh
         self.phis = np.ascontiguousarray(state.phis, dtype=np.float64)
+        if op == "weno_ext" and ig_ext is None:
+            ig_ext = mesh.ig_ext()
+        self.ig_ext = None if ig_ext is None else np.ascontiguousarray(ig_ext, dtype=np.float64)
         self.m = OracleMesh(mesh.n, mesh.halo, mesh.dx, float(dt), _p(mesh.sqrtG), _p(mesh.G), _p(mesh.IG),
-                            _p(mesh.A), _p(mesh.IA), _p(mesh.f), _p(self.phis))
+                            _p(mesh.A), _p(mesh.IA), _p(mesh.f), _p(self.phis), OPS[op], 0, _p(self.ig_ext))
         self.a = {}
         for k in ("phiG", "u", "v", "phi", "uc", "vc"):
             self.a[k] = np.ascontiguousarray(getattr(state, k), dtype=np.float64).copy()

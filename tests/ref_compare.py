@@ -4,8 +4,10 @@ Test infrastructure.  Loads oracle/_ref/libccs_ref.so (the reference's
 constants/parameters/mesh/stat/tend/projection/ghost/spatial_operators/diag/
 test_case modules compiled from their sources, see oracle/Makefile `ref`)
 and oracle/build/libccs_oracle.so, runs both on the same product mesh, and
-prints one JSON line of comparison results.  Run in its own process
-(`python tests/ref_compare.py N HALO CASE NSTEPS [plant]`): the reference allocates
+prints one JSON line of comparison results.  With op=NAME the reference library
+built with that spatial operator source (libccs_ref_{2nd,4th,6th,WENO}.so) is
+compared with the oracle's matching operator.  Run in its own process
+(`python tests/ref_compare.py N HALO CASE NSTEPS [plant] [op=NAME]`): the reference allocates
 its automatic arrays on the stack, so the work runs on a thread with a large
 stack, and each process initialises the reference modules once.
 """
@@ -22,6 +24,9 @@ ROOT = os.path.dirname(HERE)
 sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libccs_ref.so")
+# oracle operator name -> reference library built with that operator source
+REF_LIBS = {"weno": "libccs_ref.so", "centred2": "libccs_ref_2nd.so", "centred4": "libccs_ref_4th.so",
+            "centred6": "libccs_ref_6th.so", "weno_ext": "libccs_ref_WENO.so"}
 
 _D = ctypes.POINTER(ctypes.c_double)
 
@@ -30,14 +35,16 @@ def p(a):
     return a.ctypes.data_as(_D)
 
 
-def run(n, h, case_num, nsteps, out, plant=False):
+def run(n, h, case_num, nsteps, out, plant=False, op="weno"):
     import conformal_cubed_sphere_amd as ccs
     import oracle_lib
 
-    ref = ctypes.CDLL(REF_LIB)
+    ref = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", REF_LIBS[op]))
     ref.ref_init.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_size_t]
-    ref.ref_weno.restype = ctypes.c_double
-    ref.ref_weno.argtypes = [_D, ctypes.c_int]
+    has_weno = op in ("weno", "weno_ext")
+    if has_weno:
+        ref.ref_weno.restype = ctypes.c_double
+        ref.ref_weno.argtypes = [_D, ctypes.c_int]
     ref.ref_rk3.argtypes = [ctypes.c_int, ctypes.c_double]
     ref.ref_diag.argtypes = [ctypes.c_int, _D, _D]
 
@@ -47,12 +54,15 @@ def run(n, h, case_num, nsteps, out, plant=False):
     ref.ref_set_mesh(p(mesh.sqrtG), p(mesh.G), p(mesh.IG), p(mesh.A), p(mesh.IA), p(mesh.f), p(zero),
                      p(mesh.lon), p(mesh.lat), p(mesh.sinlat), p(mesh.coslat))
     inner = ~mesh.corner_mask().ravel()
-    res = {}
+    res = {"op": op}
+    ig_ext = mesh.ig_ext() if op == "weno_ext" else None
+    if ig_ext is not None:
+        ref.ref_set_mesh_ext(p(ig_ext))
 
     # WENO_limiter on random windows of several scales, both directions
     rng = np.random.default_rng(n * 100 + case_num)
     worst = 0
-    for _ in range(4000):
+    for _ in range(4000 if has_weno else 0):
         q = rng.normal(size=5) * 10.0 ** rng.integers(-4, 19)
         for d in (1, -1):
             a = ref.ref_weno(p(q), d)
@@ -80,7 +90,7 @@ def run(n, h, case_num, nsteps, out, plant=False):
     ref.ref_set_state(0, *(p(getattr(prod, k)) for k in ("phiG", "u", "v", "phi", "uc", "vc")))
 
     # tendency of the initial state
-    orc = oracle_lib.Oracle(mesh, prod, 60.0)
+    orc = oracle_lib.Oracle(mesh, prod, 60.0, op=op, ig_ext=ig_ext)
     to = orc.tendency()
     tr = [np.zeros(mesh.npad) for _ in range(3)]
     ref.ref_spatial_operator(0, *(p(x) for x in tr))
@@ -90,7 +100,7 @@ def run(n, h, case_num, nsteps, out, plant=False):
 
     # nsteps of RK3 (reference spatial operator + ghost pieces) vs the oracle
     dt, _ = ccs.cfl_time_step(mesh, prod, 0.5, 86400.0)
-    orc = oracle_lib.Oracle(mesh, prod, dt)
+    orc = oracle_lib.Oracle(mesh, prod, dt, op=op, ig_ext=ig_ext)
     if plant:  # one ulp in one interior cell of the oracle's input only
         d = (n // 2 + mesh.halo) + mesh.P * ((n // 3 + mesh.halo) + mesh.P * 2)
         orc.a["phiG"][d] = np.nextafter(orc.a["phiG"][d], np.inf)
@@ -112,13 +122,14 @@ def run(n, h, case_num, nsteps, out, plant=False):
 
 def main():
     n, h, case_num, nsteps = (int(x) for x in sys.argv[1:5])
-    plant = len(sys.argv) > 5 and sys.argv[5] == "plant"
+    plant = "plant" in sys.argv[5:]
+    op = next((x[3:] for x in sys.argv[5:] if x.startswith("op=")), "weno")
     out = {}
     err = []
 
     def target():
         try:
-            run(n, h, case_num, nsteps, out, plant)
+            run(n, h, case_num, nsteps, out, plant, op)
         except Exception as exc:  # report in the JSON line
             err.append(repr(exc))
 

@@ -99,7 +99,10 @@ def test_create_rejects_invalid_configs_before_touching_a_device():
     from conformal_cubed_sphere_amd import _lib
 
     h = ctypes.c_void_p()
-    for bad in (dict(n=3), dict(halo=2), dict(precision=9), dict(scheme=5), dict(dt=0.0), dict(eflux=1)):
+    for bad in (dict(n=3), dict(halo=2), dict(precision=9), dict(scheme=5), dict(dt=0.0), dict(eflux=1),
+                dict(spatial_op=5), dict(spatial_op=-1),
+                # the operator variants run on the FP64 parity path only
+                dict(spatial_op=1, precision=1), dict(spatial_op=4, precision=2)):
         cfg = _lib.CcsConfig(n=32, halo=3, precision=0, scheme=0, eflux=0, device=0, use_graph=0, dt=60.0,
                              dx=0.05, radius=6.371e6, gravity=9.80616, omega=7.292e-5)
         for k, v in bad.items():
@@ -163,3 +166,11 @@ def test_ensemble_rejects_rk4_until_its_gpu_fault_is_explained():
                                  persistent=0, radius=6.371e6, gravity=9.80616, omega=7.292e-5)
     assert _lib.lib.ccs_ensemble_create(ctypes.byref(cfg), n, _lib.dptr(np.array([60.0])),
                                         _lib.dptr(np.array([0.05])), ctypes.byref(h)) == _lib.CCS_ERR_INVALID
+
+
+def test_set_mesh_ext_rejects_null_and_handles():
+    from conformal_cubed_sphere_amd import _lib
+
+    assert _lib.lib.ccs_set_mesh_ext(None, None) == _lib.CCS_ERR_INVALID
+    assert _lib.lib.ccs_mesh_ext_metric(5, 1.0, None, _lib.dptr(np.zeros(4))) == _lib.CCS_ERR_INVALID
+    assert _lib.lib.ccs_mesh_ext_metric(8, 1.0, None, None) == _lib.CCS_ERR_INVALID

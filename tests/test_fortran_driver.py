@@ -33,7 +33,7 @@ NAMELIST = """&time_settings
 /
 
 &dynamic_opt
- reconstruct_scheme = 'WENO'
+ reconstruct_scheme = '{recon}'
 /
 
 &gpu_opt
@@ -43,7 +43,7 @@ NAMELIST = """&time_settings
 
 
 def _run(tmp_path, timeout=120, **kw):
-    args = dict(dt=600.0, hours=6, hist=3600, scheme="RK3_TVD", case=7, halo=3, gpu=" nx = 32")
+    args = dict(dt=600.0, hours=6, hist=3600, scheme="RK3_TVD", case=7, halo=3, gpu=" nx = 32", recon="WENO")
     args.update(kw)
     (tmp_path / "namelist.input").write_text(NAMELIST.format(**args))
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=timeout)
@@ -60,6 +60,7 @@ def _need_exe():
     (dict(gpu=" precision = 'fp64'"), "set nx"),
     (dict(gpu=" nx = 33"), "set nx"),
     (dict(halo=2), "xhalo >= 3"),
+    (dict(recon="MUSCL"), "reconstruct_scheme must be"),
 ])
 def test_namelist_validation(tmp_path, bad, msg):
     _need_exe()
@@ -143,7 +144,8 @@ def test_driver_reads_a_mesh_file(tmp_path, ccs):
     _need_exe()
     ccs.write_mesh_file(str(tmp_path / "mesh_C16.nc"), 16)
     (tmp_path / "namelist.input").write_text(NAMELIST.format(
-        dt=600.0, hours=1, hist=3600, scheme="RK3_TVD", case=2, halo=3, gpu=" precision = 'fp64'").replace(
+        dt=600.0, hours=1, hist=3600, scheme="RK3_TVD", case=2, halo=3, gpu=" precision = 'fp64'",
+        recon="WENO").replace(
         "unused.nc", "mesh_C16.nc"))
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     out = r.stdout + r.stderr
@@ -155,3 +157,25 @@ def test_driver_reads_a_mesh_file(tmp_path, ccs):
         assert r.returncode == 2
     else:
         assert r.returncode == 0, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("recon,op", [("4th", "centred4"), ("WENO_ext", "weno_ext")])
+def test_driver_operator_variants_match_oracle_bitwise(tmp_path, ccs, orc, recon, op):
+    """&dynamic_opt reconstruct_scheme selects the spatial_operators_mod
+    variant; 2 hours of C32 through the Fortran host equal the oracle's."""
+    _need_exe()
+    rc, out = _run(tmp_path, hours=2, recon=recon,
+                   gpu=" nx = 32\n cfl = 0.5\n seed = 5\n mountain = 1\n final_state_file = 'final.bin'\n"
+                       " history_file = ''")
+    print(out)
+    assert rc == 0, out
+    assert f"reconstruct_scheme {recon}" in out
+    m = ccs.Mesh(32, halo=3)
+    n, h, nsteps, dt, fin = _read_final(tmp_path / "final.bin", m.npad)
+    s0 = ccs.init_case(m, case_num=7, seed=5, mountain=True)
+    o = orc.Oracle(m, s0, dt, op=op)
+    o.steps(nsteps)
+    mask = ~m.corner_mask().ravel()
+    for k in ("phiG", "u", "v", "phi"):
+        assert np.array_equal(fin[k][mask], o.a[k][mask]), k

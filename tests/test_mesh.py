@@ -169,3 +169,34 @@ def test_product_metrics_match_oracle_restatement(ccs, orc):
         assert o["status"] == 0 and o["dx"] == m.dx
         for k in ("lon", "lat", "sinlat", "coslat", "sqrtG", "f", "areaCell", "G", "IG", "A", "IA"):
             assert np.array_equal(getattr(m, k), o[k]), k
+
+
+def test_extended_lattice_metric(ccs, mesh32):
+    # mesh%matrixIG_ext (io_mod.F90:223-250) on the (2n+1)^2 lattice
+    n, m = 32, 65
+    ig = mesh32.ig_ext().reshape(6, m, m, 2, 2)  # [p, b, a, c, r]
+    corners = np.zeros((m, m), bool)
+    corners[[0, 0, -1, -1], [0, -1, 0, -1]] = True
+    # the four corner points per panel are never set by the reference
+    assert np.all(ig[:, corners] == 0.0)
+    ok = ig[:, ~corners]
+    # symmetric and conformal: G^-1 = I / sqrtG (unit sphere scaled by a^2)
+    assert np.array_equal(ok[..., 0, 1], ok[..., 1, 0])
+    assert np.max(np.abs(ok[..., 0, 1] * mesh32.radius ** 2)) < 1e-9
+    assert np.max(np.abs(ok[..., 0, 0] / ok[..., 1, 1] - 1.0)) < 1e-9
+    # at even lattice indices (cell centres) it is the cell metric matrixIG
+    # (the two conformal-map solves agree to round-off)
+    P, h = mesh32.P, mesh32.halo
+    cell = mesh32.IG.reshape(6, P, P, 2, 2)[:, h:h + n, h:h + n]
+    cen = ig[:, 1::2, 1::2]
+    assert np.max(np.abs(cen - cell)) / np.max(np.abs(cell)) < 1e-10
+
+
+def test_extended_lattice_metric_from_mesh_file(ccs, tmp_path):
+    path = str(tmp_path / "mesh_C16.nc")
+    ccs.write_mesh_file(path, 16)
+    n, jab_ext = ccs.read_mesh_file_ext(path)
+    assert n == 16 and jab_ext.size == 36 * 33 * 33
+    mesh = ccs.Mesh(16, halo=3)
+    # the file holds this library's own map values, so the tables agree exactly
+    assert np.array_equal(mesh.ig_ext(jab_ext), mesh.ig_ext())
