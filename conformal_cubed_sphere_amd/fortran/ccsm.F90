@@ -195,8 +195,8 @@ program ccsm
     print *, 'dt from CFL ', cfl, ': ', dt, ' s, ', nsteps, ' steps'
   end if
   print *, ''
-  print *, 'Temporal integration scheme is ' // trim(adjustl(integral_scheme)), ', precision ', trim(precision), &
-           ', reconstruct_scheme ', trim(adjustl(reconstruct_scheme))
+  write(*, '(1x,6a)') 'Temporal integration scheme is ', trim(adjustl(integral_scheme)), ', precision ', &
+                      trim(precision), ', reconstruct_scheme ', trim(adjustl(reconstruct_scheme))
   print *, ''
 
   ! ---- GPU state (initStat/initTend)
