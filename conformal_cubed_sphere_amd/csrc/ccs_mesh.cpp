@@ -218,6 +218,29 @@ cd w_of_z(cd zt, cd guess, int* iters) {
   return w;
 }
 
+// Edge x = 1 of the face, z = 1 + i yt with 0 <= yt < 1.  The edge is mapped
+// onto the stereographic image of the edge great circle, w = sqrt(2) e^{it} - 1,
+// on which Re z = 1 holds.  There z_of_w's elliptic integral sits on its branch
+// cut, so Im z comes out with either sign (the magnitude is right): solve
+// |Im z(w(t))| = yt by Newton in the angle t, with
+// dz/dt = dz/dw * dw/dt = i sqrt(2) e^{it} / (kappa W(w)^{1/4}).  (The 2-D
+// Newton of w_of_z, which must stay inside the circle, stalls along it.)
+cd w_on_edge(double yt, cd guess) {
+  const double k = kappa(), r2 = std::sqrt(2.0);
+  double t = std::arg(guess + 1.0);
+  int polish = -1;
+  for (int it = 0; it < 60; ++it) {
+    const cd e = std::polar(1.0, t);
+    const cd w = r2 * e - 1.0;
+    const cd dzdt = cd(0.0, r2) * e / (k * std::pow(face_form(w), 0.25));
+    const double dt = (std::fabs(z_of_w(w).imag()) - yt) / std::fabs(dzdt.imag());
+    t -= dt;
+    if (polish < 0 && std::fabs(dt) <= 1e-15) polish = 2;  // two more steps
+    if (polish >= 0 && polish-- == 0) break;
+  }
+  return r2 * std::polar(1.0, t) - 1.0;
+}
+
 // Panel frames: global xc = R_p * local xc, local face centre (0,0,1),
 // local x toward +Re w, local y toward +Im w.  Columns as in the reference
 // (mesh_generation/src/cuco.f90:100-105), stored here as R[p][row][col].
@@ -251,14 +274,13 @@ void map_points(const std::vector<double>& coord, std::vector<cd>& w, std::vecto
     cd guess = diag_guess;
     for (int i = j; i < m; ++i) {
       const cd zt(coord[i], coord[j]);
-      cd wi = (coord[i] == 1.0 && coord[j] == 1.0) ? vertex : w_of_z(zt, guess, nullptr);
-      if (coord[i] == 1.0 && coord[j] != 1.0) {
-        // panel edge x = 1: w lies on the stereographic image of the edge
-        // great circle, |w + 1| = sqrt(2); project the Newton result onto it
-        // so that both panels sharing the edge map it to the same points
-        const cd c = wi + 1.0;
-        wi = std::sqrt(2.0) * c / std::abs(c) - 1.0;
-      }
+      cd wi;
+      if (coord[i] == 1.0 && coord[j] == 1.0)
+        wi = vertex;
+      else if (coord[i] == 1.0)  // panel edge x = 1: a 1-D solve on the edge circle
+        wi = w_on_edge(coord[j], guess);
+      else
+        wi = w_of_z(zt, guess, nullptr);
       tri[static_cast<std::size_t>(j) * m + i] = wi;
       if (i == j) {
         // extrapolate along the diagonal for the next row's start
