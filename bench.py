@@ -77,16 +77,47 @@ def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno"):
     return core, wall, ev_total, kern_ms
 
 
-def cpu_baseline(ccs, mesh, s0, dt, budget_s):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib  # the checker, timed here as the CPU baseline
-
+def host_cores():
     cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return int(os.environ.get("OMP_NUM_THREADS", cores))
+
+
+def cpu_baseline_reference(n, budget_s):
+    """The reference's own step (oracle/_ref/libccs_ref_omp.so: its modules
+    built from source with its GNU flags, OpenMP on), timed in a child
+    process by oracle/ref_timing.py; None when that library is not built."""
+    import subprocess
+
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libccs_ref_omp.so")):
+        return None
+    cores = host_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    try:
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "ref_timing.py"), str(n), str(budget_s)],
+                           capture_output=True, text=True, timeout=30 * budget_s + 120, env=env)
+        line = [x for x in r.stdout.splitlines() if x.startswith("{")]
+        res = json.loads(line[-1]) if line else {"error": r.stderr[-300:]}
+    except (subprocess.TimeoutExpired, ValueError) as exc:
+        res = {"error": repr(exc)}
+    if "error" in res:
+        return None
+    return {"value": res["value"], "unit": "cell-updates/s", "cores": cores, "kind": "reference",
+            "sample": f"the reference's dynamic-core modules (spatial_operators_mod, ghost_mod, projection_mod, "
+                      f"... compiled from source with -O3 -fdefault-real-8 -fopenmp) in an RK3_TVD loop "
+                      f"(oracle/ref_harness.F90, oracle/ref_timing.py) on the same C{n} workload: "
+                      f"{res['steps']} SSP-RK3 steps after 1 untimed step, {res['seconds']:.1f} s on {cores} "
+                      f"OpenMP threads"}
+
+
+def cpu_baseline(ccs, mesh, s0, dt, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib  # the checker, timed here as the CPU baseline
+
+    cores = host_cores()
     orc = oracle_lib.Oracle(mesh, s0, dt, threads=cores)
     t0 = time.perf_counter()
     orc.steps(1)
@@ -270,7 +301,11 @@ def main():
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(ccs, args.ens_members, args.ens_steps, 5)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ccs, mesh, s0, dt, args.cpu_seconds)
+        port = cpu_baseline(ccs, mesh, s0, dt, args.cpu_seconds)
+        ref = cpu_baseline_reference(args.n, args.cpu_seconds)
+        out["cpu_baseline"] = ref if ref is not None else port
+        if ref is not None:
+            out["cpu_oracle_port"] = port  # the C restatement, for comparison
     print(json.dumps(out), flush=True)
 
 
