@@ -374,6 +374,9 @@ int read_mesh(const char* path, int* n_out, double* lon_deg, double* lat_deg, do
   };
   double ids = 0, ide = 0, jds = 0, jde = 0;
   if (!gatt("ids", &ids) || !gatt("ide", &ide) || !gatt("jds", &jds) || !gatt("jde", &jde)) return CCS_ERR_INVALID;
+  // lattice extents from the file: keep the int conversions in range
+  constexpr double kMaxExt = 1 << 20;
+  if (!(ide - ids >= 8 && ide - ids <= kMaxExt && jde - jds >= 8 && jde - jds <= kMaxExt)) return CCS_ERR_INVALID;
   const int n = static_cast<int>(ide - ids) / 2;  // io_mod.F90:49-52
   if (n < 4 || static_cast<int>(jde - jds) / 2 != n) return CCS_ERR_INVALID;
   *n_out = n;
@@ -389,8 +392,11 @@ int read_mesh(const char* path, int* n_out, double* lon_deg, double* lat_deg, do
   if (!vl || !vt || !vj || vl->type != NC_DOUBLE || vt->type != NC_DOUBLE || vj->type != NC_DOUBLE)
     return CCS_ERR_INVALID;
   const std::size_t np = static_cast<std::size_t>(m) * m;
-  if (vl->begin + 6 * np * 8 > r.d.size() || vt->begin + 6 * np * 8 > r.d.size() ||
-      vj->begin + 36 * np * 8 > r.d.size())
+  // data ranges inside the file (offsets come from the file: no wrap-around)
+  auto fits = [&r](uint64_t begin, std::size_t bytes) {
+    return begin <= r.d.size() && bytes <= r.d.size() - begin;
+  };
+  if (!fits(vl->begin, 6 * np * 8) || !fits(vt->begin, 6 * np * 8) || !fits(vj->begin, 36 * np * 8))
     return CCS_ERR_INVALID;
   // cell centres at extended indices 2i, 2j (1-based; io_mod.F90:122-135)
   for (int p = 0; p < 6; ++p)

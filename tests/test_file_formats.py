@@ -100,3 +100,23 @@ def test_history_file_schema_and_records(tmp_path, ccs):
     corner = mesh.corner_mask().reshape(6, P, P)
     assert np.all(lon[corner] == -9999999999999999.0)
     assert np.allclose(lon[~corner].ravel(), np.degrees(mesh.lon.reshape(6, P, P)[~corner]))
+
+
+def test_mesh_file_reader_rejects_truncated_and_out_of_range_files(tmp_path, ccs):
+    path = tmp_path / "mesh.nc"
+    ccs.write_mesh_file(str(path), 8)
+    data = path.read_bytes()
+    # data section cut short: every variable's byte range must lie inside the file
+    cut = tmp_path / "cut.nc"
+    cut.write_bytes(data[: len(data) // 2])
+    with pytest.raises(ccs.CcsError):
+        ccs.read_mesh_file(str(cut))
+    # lattice extent far out of range (ide = 2^31 - 1)
+    k = data.index(b"ide\x00")
+    # name (padded to 4 bytes), then type NC_INT (4), count 1, value
+    off = k + 4 + 8
+    assert data[k + 4:k + 12] == b"\x00\x00\x00\x04\x00\x00\x00\x01"
+    big = tmp_path / "big.nc"
+    big.write_bytes(data[:off] + b"\x7f\xff\xff\xff" + data[off + 4:])
+    with pytest.raises(ccs.CcsError):
+        ccs.read_mesh_file(str(big))
