@@ -61,3 +61,4 @@ def test_split_small_and_ragged_grids(ccs, n, h):
     print(f"  L2(h) fp32 vs fp64 {l2:.2e}, fp32 mass drift {drift32:.2e}")
     assert l2 < 1e-5
     assert abs(drift32) < 1e-8
+
