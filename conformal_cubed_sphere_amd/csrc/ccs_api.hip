@@ -46,11 +46,16 @@ struct ccs_handle {
   double* face_speed = nullptr;  // CCS_OP_WENO_EXT: x-face then y-face G^-1 entries
   // split-precision path
   SplitState* split = nullptr;
-  // graphs of one step ([0]) and of kGraphSteps steps ([1]); every step
-  // leaves the state in the same buffer, so a graph of several steps is the
-  // one-step sequence repeated
-  hipGraph_t graph[2]{};
-  hipGraphExec_t graph_exec[2]{};
+  // graphs of one step ([0]), of kGraphSteps steps ending with a rebase ([1])
+  // and of one step followed by a rebase ([2]); every step leaves the state
+  // in the same buffer, so a graph of several steps is the one-step sequence
+  // repeated
+  hipGraph_t graph[3]{};
+  hipGraphExec_t graph_exec[3]{};
+  // steps since the last ccs_set_state: the split path rebases its
+  // geopotential deviation after every step whose count is a multiple of
+  // kGraphSteps, whatever the grouping of steps into calls
+  long long steps_done = 0;
 };
 
 namespace {
@@ -135,6 +140,34 @@ int enqueue_step(ccs_handle* hd, bool expl, hipEvent_t* ev) {
 }
 
 int stages_per_step(const ccs_handle* hd) { return hd->cfg.scheme == CCS_RK3_TVD ? 3 : 4; }
+
+bool rebase_after(const ccs_handle* hd, long long step_index) {  // step_index: 1-based count after the step
+  return hd->cfg.precision != CCS_FP64_PARITY && step_index % kGraphSteps == 0;
+}
+
+// one step, then the rebase if the step count reaches a multiple of kGraphSteps
+int enqueue_counted_step(ccs_handle* hd, bool expl, hipEvent_t* ev) {
+  const int rc = enqueue_step(hd, expl, ev);
+  if (rc != CCS_OK) return rc;
+  ++hd->steps_done;
+  if (rebase_after(hd, hd->steps_done)) split_enqueue_rebase(hd->split, hd->stream);
+  return CCS_OK;
+}
+
+int capture_graph(ccs_handle* hd, int g) {
+  if (hd->graph_exec[g]) return CCS_OK;
+  HIPCHK(hd, hipStreamBeginCapture(hd->stream, hipStreamCaptureModeThreadLocal));
+  int rc = CCS_OK;
+  for (int s = 0; s < (g == 1 ? kGraphSteps : 1) && rc == CCS_OK; ++s) rc = enqueue_step(hd, false, nullptr);
+  if (rc == CCS_OK && g != 0 && hd->cfg.precision != CCS_FP64_PARITY) split_enqueue_rebase(hd->split, hd->stream);
+  hipGraph_t gr = nullptr;
+  const hipError_t ec = hipStreamEndCapture(hd->stream, &gr);
+  if (rc != CCS_OK) return rc;
+  if (ec != hipSuccess) return fail(hd, CCS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  hd->graph[g] = gr;
+  HIPCHK(hd, hipGraphInstantiate(&hd->graph_exec[g], hd->graph[g], nullptr, nullptr, 0));
+  return CCS_OK;
+}
 
 }  // namespace
 
@@ -223,7 +256,7 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
 
 extern "C" void ccs_destroy(ccs_handle* hd) {
   if (!hd) return;
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 3; ++g) {
     if (hd->graph_exec[g]) hipGraphExecDestroy(hd->graph_exec[g]);
     if (hd->graph[g]) hipGraphDestroy(hd->graph[g]);
   }
@@ -346,6 +379,7 @@ extern "C" int ccs_set_state(ccs_handle* hd, const double* phiG, const double* u
   }
   hd->explicit_valid = expl;
   hd->state_set = true;
+  hd->steps_done = 0;
   return CCS_OK;
 }
 
@@ -358,31 +392,29 @@ extern "C" int ccs_step(ccs_handle* hd, int nsteps) {
   HIPCHK(hd, hipSetDevice(hd->cfg.device));
   int k = 0;
   if (nsteps > 0 && hd->explicit_valid) {
-    const int rc = enqueue_step(hd, true, nullptr);
+    const int rc = enqueue_counted_step(hd, true, nullptr);
     if (rc != CCS_OK) return rc;
     HIPCHK(hd, hipGetLastError());
     hd->explicit_valid = false;
     k = 1;
   }
-  if (k < nsteps && hd->cfg.use_graph) {
-    const int want = (nsteps - k >= kGraphSteps) ? 1 : 0;
-    for (int g = 0; g <= want; ++g) {
-      if (hd->graph_exec[g]) continue;
-      HIPCHK(hd, hipStreamBeginCapture(hd->stream, hipStreamCaptureModeThreadLocal));
-      int rc = CCS_OK;
-      for (int s = 0; s < (g ? kGraphSteps : 1) && rc == CCS_OK; ++s) rc = enqueue_step(hd, false, nullptr);
-      hipGraph_t gr = nullptr;
-      const hipError_t ec = hipStreamEndCapture(hd->stream, &gr);
+  if (hd->cfg.use_graph) {
+    // 16-step graphs on steps aligned to multiples of kGraphSteps (each ends
+    // with the rebase), single steps elsewhere
+    while (k < nsteps) {
+      int g;
+      if (hd->steps_done % kGraphSteps == 0 && nsteps - k >= kGraphSteps) g = 1;
+      else g = rebase_after(hd, hd->steps_done + 1) ? 2 : 0;
+      const int rc = capture_graph(hd, g);
       if (rc != CCS_OK) return rc;
-      if (ec != hipSuccess) return fail(hd, CCS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-      hd->graph[g] = gr;
-      HIPCHK(hd, hipGraphInstantiate(&hd->graph_exec[g], hd->graph[g], nullptr, nullptr, 0));
+      HIPCHK(hd, hipGraphLaunch(hd->graph_exec[g], hd->stream));
+      const int adv = (g == 1) ? kGraphSteps : 1;
+      hd->steps_done += adv;
+      k += adv;
     }
-    for (; k + kGraphSteps <= nsteps; k += kGraphSteps) HIPCHK(hd, hipGraphLaunch(hd->graph_exec[1], hd->stream));
-    for (; k < nsteps; ++k) HIPCHK(hd, hipGraphLaunch(hd->graph_exec[0], hd->stream));
   }
   for (; k < nsteps; ++k) {
-    const int rc = enqueue_step(hd, false, nullptr);
+    const int rc = enqueue_counted_step(hd, false, nullptr);
     if (rc != CCS_OK) return rc;
   }
   HIPCHK(hd, hipGetLastError());
@@ -501,7 +533,7 @@ extern "C" int ccs_time_steps(ccs_handle* hd, int nsteps, double* total_ms, doub
     for (auto& e : ev) HIPCHK(hd, hipEventCreate(&e));
     double acc = 0.0;
     for (int k = 0; k < nsteps && rc == CCS_OK; ++k) {
-      rc = enqueue_step(hd, false, ev.data());
+      rc = enqueue_counted_step(hd, false, ev.data());
       HIPCHK(hd, hipEventSynchronize(ev[2 * ns - 1]));
       for (int s = 0; s < ns; ++s) {
         float t = 0.f;
@@ -528,8 +560,9 @@ struct ccs_ensemble {
   EnsPlan* plan = nullptr;
   int tile_y = 0, persistent = 0;
   double stage_ms = 0.0;
-  hipGraph_t graph[2]{};
-  hipGraphExec_t graph_exec[2]{};
+  hipGraph_t graph[3]{};       // as ccs_handle: one step, 16 steps + rebase, one step + rebase
+  hipGraphExec_t graph_exec[3]{};
+  long long steps_done = 0;    // steps since the last ccs_ensemble_set_state
 };
 
 namespace {
@@ -547,7 +580,7 @@ int efail(ccs_ensemble* e, int code, const std::string& msg) {
   } while (0)
 
 void ens_drop_graphs(ccs_ensemble* e) {
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 3; ++g) {
     if (e->graph_exec[g]) (void)hipGraphExecDestroy(e->graph_exec[g]);
     if (e->graph[g]) (void)hipGraphDestroy(e->graph[g]);
     e->graph_exec[g] = nullptr;
@@ -651,6 +684,7 @@ extern "C" int ccs_ensemble_set_state(ccs_ensemble* e, int member, const double*
   int rc = ens_member(e, member);
   if (rc != CCS_OK) return rc;
   ENSCHK(e, hipStreamSynchronize(e->stream));
+  e->steps_done = 0;  // rebase points count from the last upload
   return ens_forward(e, member, ccs_set_state(e->m[member], phiG, u, v, nullptr, nullptr, nullptr));
 }
 
@@ -744,22 +778,32 @@ extern "C" int ccs_ensemble_step(ccs_ensemble* e, int nsteps) {
   ENSCHK(e, hipSetDevice(e->cfg.device));
   // members' own streams (uploads) are idle: every member call synchronises
   const int scheme = e->cfg.scheme;
+  auto rebase_due = [](long long step_index) { return step_index % kGraphSteps == 0; };
   int k = 0;
-  if (e->cfg.use_graph && nsteps > 0) {
-    const int want = (nsteps >= kGraphSteps) ? 1 : 0;
-    for (int g = 0; g <= want; ++g) {
-      if (e->graph_exec[g]) continue;
-      ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      for (int s = 0; s < (g ? kGraphSteps : 1); ++s) ens_enqueue_step(e->plan, e->stream, scheme);
-      hipGraph_t gr = nullptr;
-      ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
-      e->graph[g] = gr;
-      ENSCHK(e, hipGraphInstantiate(&e->graph_exec[g], e->graph[g], nullptr, nullptr, 0));
+  if (e->cfg.use_graph) {
+    while (k < nsteps) {
+      int g;
+      if (e->steps_done % kGraphSteps == 0 && nsteps - k >= kGraphSteps) g = 1;
+      else g = rebase_due(e->steps_done + 1) ? 2 : 0;
+      if (!e->graph_exec[g]) {
+        ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < (g == 1 ? kGraphSteps : 1); ++s) ens_enqueue_step(e->plan, e->stream, scheme);
+        if (g != 0) ens_enqueue_rebase(e->plan, e->stream);
+        hipGraph_t gr = nullptr;
+        ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
+        e->graph[g] = gr;
+        ENSCHK(e, hipGraphInstantiate(&e->graph_exec[g], e->graph[g], nullptr, nullptr, 0));
+      }
+      ENSCHK(e, hipGraphLaunch(e->graph_exec[g], e->stream));
+      const int adv = (g == 1) ? kGraphSteps : 1;
+      e->steps_done += adv;
+      k += adv;
     }
-    for (; k + kGraphSteps <= nsteps; k += kGraphSteps) ENSCHK(e, hipGraphLaunch(e->graph_exec[1], e->stream));
-    for (; k < nsteps; ++k) ENSCHK(e, hipGraphLaunch(e->graph_exec[0], e->stream));
   }
-  for (; k < nsteps; ++k) ens_enqueue_step(e->plan, e->stream, scheme);
+  for (; k < nsteps; ++k) {
+    ens_enqueue_step(e->plan, e->stream, scheme);
+    if (rebase_due(++e->steps_done)) ens_enqueue_rebase(e->plan, e->stream);
+  }
   ENSCHK(e, hipGetLastError());
   return CCS_OK;
 }

@@ -22,6 +22,9 @@ int split_set_mesh(SplitState* s, hipStream_t st, double* const T[15], std::stri
 int split_set_state(SplitState* s, hipStream_t st, const double* phiG, const double* u, const double* v,
                     const double* phi, const double* uc, const double* vc, std::string* err);
 int split_enqueue_step(SplitState* s, hipStream_t st, int scheme, bool first, hipEvent_t* ev);
+// Fold the geopotential deviation into the FP64 reference field (state at a
+// step boundary, buffer 0); the represented state is unchanged.
+void split_enqueue_rebase(SplitState* s, hipStream_t st);
 int split_get_state(SplitState* s, hipStream_t st, bool first, double* phiG, double* u, double* v,
                     double* phi, double* uc, double* vc, double* zonal, double* merid, std::string* err);
 int split_tendency(SplitState* s, hipStream_t st, bool first, double* tp, double* tu, double* tv,
@@ -37,6 +40,7 @@ int ens_plan_create(SplitState* const* mem, int nm, int tile_y, int persistent, 
                     std::string* err);
 void ens_plan_destroy(EnsPlan* e);
 void ens_enqueue_step(const EnsPlan* e, hipStream_t st, int scheme);
+void ens_enqueue_rebase(const EnsPlan* e, hipStream_t st);
 void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st);
 int ens_plan_grid(const EnsPlan* e);
 int ens_plan_tile_y(const EnsPlan* e);

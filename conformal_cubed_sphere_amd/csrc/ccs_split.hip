@@ -528,6 +528,9 @@ struct EnsMember {
   T inc_half, inc_full;  // RK4 increments dt/(2a), dt/a
   int tile0;             // first global tile of the member
   int hs[6][4][11];
+  double* pref64;        // FP64 reference field and its cast (rebase)
+  T* pref_rw;
+  unsigned long long np; // padded cells
 };
 
 struct EnsStage {
@@ -578,6 +581,35 @@ __global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restr
     int tid = static_cast<int>(threadIdx.x);
     asm volatile("" : "+v"(tid));
     ens_tile<T, KIND, TY>(md, tile_member, sel, t, tid);
+  }
+}
+
+// Rebase of the geopotential deviation: the exact sum Pref + dP becomes the
+// new FP64 reference field, its cast the FP32 one, and dP restarts at zero.
+// The represented state is unchanged (in FP64); keeping dP small keeps the
+// rounding of each stage's increment relative to the increment, so the FP32
+// state stops losing mass to round-off accumulated into a growing deviation.
+template <typename T>
+__global__ void __launch_bounds__(256) split_rebase(double* __restrict__ pref64, T* __restrict__ pref,
+                                                    T* __restrict__ dP, unsigned long long np) {
+  for (unsigned long long d = blockIdx.x * 256ull + threadIdx.x; d < np; d += gridDim.x * 256ull) {
+    const double p = pref64[d] + static_cast<double>(dP[d]);
+    pref64[d] = p;
+    pref[d] = static_cast<T>(p);
+    dP[d] = T(0);
+  }
+}
+
+// the same for every member of an ensemble (blockIdx.y = member)
+template <typename T>
+__global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict__ md) {
+  const EnsMember<T>* M = md + blockIdx.y;
+  T* dP = M->buf[0][0];
+  for (unsigned long long d = blockIdx.x * 256ull + threadIdx.x; d < M->np; d += gridDim.x * 256ull) {
+    const double p = M->pref64[d] + static_cast<double>(dP[d]);
+    M->pref64[d] = p;
+    M->pref_rw[d] = static_cast<T>(p);
+    dP[d] = T(0);
   }
 }
 
@@ -930,12 +962,27 @@ int split_enqueue_step(SplitState* s, hipStream_t st, int scheme, bool, hipEvent
   return CCS_OK;
 }
 
+void split_enqueue_rebase(SplitState* s, hipStream_t st) {
+  const dim3 grid(static_cast<unsigned>(std::min<std::size_t>((s->np + 255) / 256, 2048))), block(256);
+  if (s->fp64)
+    hipLaunchKernelGGL(split_rebase<double>, grid, block, 0, st, s->pref64_d, static_cast<double*>(s->pref),
+                       static_cast<double*>(s->buf[0][0]), static_cast<unsigned long long>(s->np));
+  else
+    hipLaunchKernelGGL(split_rebase<float>, grid, block, 0, st, s->pref64_d, static_cast<float*>(s->pref),
+                       static_cast<float*>(s->buf[0][0]), static_cast<unsigned long long>(s->np));
+}
+
 int split_get_state(SplitState* s, hipStream_t st, bool, double* phiG, double* u, double* v, double* phi,
                     double* uc, double* vc, double* zonal, double* merid, std::string* err) {
   const int n = s->n, h = s->h;
   const std::size_t np = s->np;
   const double r = s->cfg.radius;
   std::vector<double> dP, U, V;
+  // the device reference field (a rebase replaces it)
+  if (hipMemcpyAsync(s->pref64.data(), s->pref64_d, np * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) {
+    *err = "split_get_state: download failed";
+    return CCS_ERR_DEVICE;
+  }
   if (s->fp64) {
     download_to_double<double>(st, s->buf[0][0], np, dP);
     download_to_double<double>(st, s->buf[0][1], np, U);
@@ -1082,6 +1129,9 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, std::strin
     for (int b = 0; b < 3; ++b)
       for (int k = 0; k < 3; ++k) d.buf[b][k] = static_cast<T*>(s->buf[b][k]);
     for (int k = 0; k < 9; ++k) d.tend[k] = static_cast<T*>(s->tend[k]);
+    d.pref64 = s->pref64_d;
+    d.pref_rw = static_cast<T*>(s->pref);
+    d.np = s->np;
     d.inc_half = static_cast<T>(0.5 * s->cfg.dt / s->cfg.radius);
     d.inc_full = static_cast<T>(s->cfg.dt / s->cfg.radius);
     d.tile0 = static_cast<int>(tiles.size());
@@ -1168,6 +1218,12 @@ void ens_plan_destroy(EnsPlan* e) {
 void ens_enqueue_step(const EnsPlan* e, hipStream_t st, int scheme) {
   if (e->fp64) ens_step_t<double>(e, st, scheme);
   else ens_step_t<float>(e, st, scheme);
+}
+
+void ens_enqueue_rebase(const EnsPlan* e, hipStream_t st) {
+  const dim3 grid(512, static_cast<unsigned>(e->nm)), block(256);
+  if (e->fp64) hipLaunchKernelGGL(ens_rebase<double>, grid, block, 0, st, static_cast<const EnsMember<double>*>(e->d_desc));
+  else hipLaunchKernelGGL(ens_rebase<float>, grid, block, 0, st, static_cast<const EnsMember<float>*>(e->d_desc));
 }
 
 void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st) {
