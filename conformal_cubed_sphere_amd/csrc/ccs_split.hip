@@ -18,9 +18,11 @@
 // 1e-2 for E, 1e-2/a^4 for phitG/a^2.
 //
 // The geopotential is carried as an FP32 deviation dP from an FP64 reference
-// field Pref (the state at the last upload); the RK stage combinations have
-// weights summing to one, so they act on dP alone and the FP32 increments are
-// not lost against the ~1e5 magnitude of P.
+// field Pref; the RK stage combinations have weights summing to one, so they
+// act on dP alone and the FP32 increments are not lost against the ~1e5
+// magnitude of P.  Pref is the uploaded state, and every 16 steps the host
+// folds dP into it (split_rebase): a deviation that kept growing would round
+// each increment against an ever larger value and lose mass.
 //
 // Kernel structure (one launch per RK stage, as the FP64 path):
 //   phase 1  pointwise terms on the tile and a 3-cell cross border -> LDS
