@@ -95,7 +95,10 @@ def test_split_fp32_at_method_sensitivity_floor(ccs, orc, n, hours):
     print(f"  perturbed vs oracle h: L1 {efl[0]:.3e} L2 {efl[1]:.3e} Linf {efl[2]:.3e}")
     print(f"  relative mass drift {(m1 - m0) / m0:.3e}")
     assert e32[1] <= 4.0 * efl[1] + 1e-9
-    assert abs(m1 - m0) <= 1e-9 * abs(m0)
+    # mass: exact flux telescoping, FP64 compensated sums and the rebase of
+    # the FP32 deviation every 16 steps leave the round-off of the increments
+    # (measured: C32 over 24 h 2.3e-10, C96 over 6 h 3.2e-11)
+    assert abs(m1 - m0) <= 5e-10 * abs(m0)
 
 
 def orc_zonal(mesh, o):
