@@ -256,15 +256,16 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
 
 extern "C" void ccs_destroy(ccs_handle* hd) {
   if (!hd) return;
+  // teardown: errors here have nowhere to go
   for (int g = 0; g < 3; ++g) {
-    if (hd->graph_exec[g]) hipGraphExecDestroy(hd->graph_exec[g]);
-    if (hd->graph[g]) hipGraphDestroy(hd->graph[g]);
+    if (hd->graph_exec[g]) (void)hipGraphExecDestroy(hd->graph_exec[g]);
+    if (hd->graph[g]) (void)hipGraphDestroy(hd->graph[g]);
   }
-  if (hd->stream) hipStreamSynchronize(hd->stream);
-  if (hd->dmem) hipFree(hd->dmem);
-  if (hd->face_speed) hipFree(hd->face_speed);
+  if (hd->stream) (void)hipStreamSynchronize(hd->stream);
+  if (hd->dmem) (void)hipFree(hd->dmem);
+  if (hd->face_speed) (void)hipFree(hd->face_speed);
   if (hd->split) split_destroy(hd->split);
-  if (hd->stream) hipStreamDestroy(hd->stream);
+  if (hd->stream) (void)hipStreamDestroy(hd->stream);
   delete hd;
 }
 
@@ -541,11 +542,11 @@ extern "C" int ccs_time_steps(ccs_handle* hd, int nsteps, double* total_ms, doub
         acc += t;
       }
     }
-    for (auto& e : ev) hipEventDestroy(e);
+    for (auto& e : ev) (void)hipEventDestroy(e);
     *kernel_ms = acc / (static_cast<double>(nsteps) * ns);
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return rc;
 }
 
