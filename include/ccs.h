@@ -197,7 +197,10 @@ int ccs_set_state(ccs_handle* h, const double* phiG, const double* u,
 /* Advance nsteps time steps: replaces the loop body
  * `call RK3_TVD(stat(new), stat(old)); call copyStat(...)`
  * (ccsm.F90:50-66, time_scheme_mod.F90:46-67) or RK4 (:13-44).
- * Asynchronous on the handle's stream. */
+ * Asynchronous on the handle's stream.  The split paths fold their FP32
+ * geopotential deviation into the FP64 reference field after every step
+ * whose count since ccs_set_state is a multiple of 16, so a run's rounding
+ * does not depend on how its steps are grouped into calls. */
 int ccs_step(ccs_handle* h, int nsteps);
 
 /* Copy the state back (padded).  zonal/merid are the sphere components on
