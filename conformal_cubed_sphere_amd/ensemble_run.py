@@ -45,8 +45,9 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64_fast"])
     ap.add_argument("--tile-y", type=int, default=0, help="8, 16, 32 or 0 (autotune)")
     ap.add_argument("--persistent", type=int, default=-1, help="0, 1 or -1 (autotune)")
+    ap.add_argument("--sizes", default="48,96,144,192", help="panel sizes the members draw from")
     args = ap.parse_args()
-    runs, ns = build_members(args.members)
+    runs, ns = build_members(args.members, sizes=tuple(int(x) for x in args.sizes.split(",")))
     cells = sum(6 * n * n for n in ns)
     with Ensemble([r[0] for r in runs], [r[2] for r in runs], precision=args.precision, tile_y=args.tile_y,
                   persistent=args.persistent) as ens:
