@@ -187,8 +187,9 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
   if (cfg->eflux != CCS_EFLUX_NONE) return CCS_ERR_INVALID;
   if (!(cfg->dt > 0.0) || !(cfg->dx > 0.0) || !(cfg->radius > 0.0)) return CCS_ERR_INVALID;
   if (cfg->spatial_op < CCS_OP_WENO || cfg->spatial_op > CCS_OP_WENO_EXT) return CCS_ERR_INVALID;
-  // the split-precision kernels implement the reference's default operator only
-  if (cfg->spatial_op != CCS_OP_WENO && cfg->precision != CCS_FP64_PARITY) return CCS_ERR_INVALID;
+  // the split-precision kernels implement the default and the centred
+  // operators; the extended-lattice metric of WENO_EXT is parity-path only
+  if (cfg->spatial_op == CCS_OP_WENO_EXT && cfg->precision != CCS_FP64_PARITY) return CCS_ERR_INVALID;
   ccs_handle* hd = new ccs_handle();
   hd->cfg = *cfg;
   hd->n = cfg->n;

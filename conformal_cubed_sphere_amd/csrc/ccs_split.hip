@@ -43,6 +43,9 @@
 //            drives long runs unstable).
 //   phase 3  divergence, vorticity, RK update, halo scatter with the signed
 //            permutation of the panel-basis change (ghost_mod.f90:100-153).
+// The centred operators of spatial_operators_mod_{2nd,4th,6th}.F90 are the
+// OP = 1/2/3 instantiations: no phase 2, and phase 3 takes centred
+// differences of the phase-1 fields.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -231,8 +234,11 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
 }
 
 // One RK stage on one 32 x TY tile of panel p with lower-left interior cell
-// (i0, j0); hr = the panel's four halo-scatter rules.
-template <typename T, int KIND, int TY>
+// (i0, j0); hr = the panel's four halo-scatter rules.  OP: the spatial
+// operator (enum ccs_spatial_op): 0 WENO-Z / Rusanov, 1/2/3 the centred
+// 2nd/4th/6th-order differences of the reference's swap-in operator modules,
+// which skip phase 2.
+template <typename T, int KIND, int TY, int OP>
 __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitStatics<T>& st, const int (*hr)[11],
                                            const int p, const int i0, const int j0, const int tid) {
 #pragma clang fp contract(fast)
@@ -254,9 +260,10 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   // face shares: x-faces [TY][TX+1], y-faces [TY+1][TX]; per face the share
   // of the flux of phitG*u (F) and of E from its left/lower side (p) and its
   // right/upper side (m)
+  constexpr bool kWeno = (OP == 0);
   constexpr int NFXF = TY * (TX + 1);
   constexpr int NFYF = (TY + 1) * TX;
-  __shared__ T sfc[4 * NFXF + 4 * NFYF];
+  __shared__ T sfc[kWeno ? 4 * NFXF + 4 * NFYF : 1];
   T* const xpF = sfc;
   T* const xmF = sfc + NFXF;
   T* const xpE = sfc + 2 * NFXF;
@@ -336,7 +343,7 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   // and Rusanov speeds stay in registers.
   const int tx = tid % TX, ty = tid / TX;
   T fr, fl, er, el, fu, fd, eu, ed, lr, ll, lu, ld;
-  {
+  if constexpr (kWeno) {
     const int l = (ty + 3) * EX + tx + 3;
     const int fx = ty * (TX + 1) + tx;   // left x-face of the cell
     const int fy = ty * TX + tx;         // bottom y-face of the cell
@@ -371,7 +378,7 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       ymE[fy] = ed;
     }
   }
-  {
+  if constexpr (kWeno) {
     // windows centred one cell outside the tile, which give only the share
     // of the tile's boundary face: x: centres -1 (right face -> x-face 0) and
     // TX (left face -> x-face TX) of each row; y: centres -1 and TY of each
@@ -415,28 +422,48 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       *oF = T(0.5) * (left ? vf - lam * vp : vf + lam * vp);
       *oE = T(0.5) * ve;
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   // ---- phase 3: cells
   const int i = i0 + tx, j = j0 + ty;
   if (i >= n || j >= n) return;
   const T rdx = a.rdx;
   const int lc = (ty + 3) * EX + (tx + 3);
-  const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
   // F(right face) - F(left face): centre terms from differences of
   // neighbouring centre values, plus the face shares.  Both cells beside a
   // face form the same share sum and the same lam (P_B - P_A) product
   // (identical operands, IEEE addition commutes), so the fluxes telescope.
-  const T pc = sPt[lc];
-  const T cx = T(0.5) * (sFx[lc + 1] - sFx[lc - 1]) - T(0.5) * (lr * (sPt[lc + 1] - pc) - ll * (pc - sPt[lc - 1]));
-  const T cy = T(0.5) * (sFy[lc + EX] - sFy[lc - EX]) - T(0.5) * (lu * (sPt[lc + EX] - pc) - ld * (pc - sPt[lc - EX]));
-  const T dFx = (cx + ((fr + xmF[fx + 1]) - (xpF[fx] + fl))) * rdx;
-  const T dEx = (T(0.5) * (sE[lc + 1] - sE[lc - 1]) + ((er + xmE[fx + 1]) - (xpE[fx] + el))) * rdx;
-  const T dFy = (cy + ((fu + ymF[fy + TX]) - (ypF[fy] + fd))) * rdx;
-  const T dEy = (T(0.5) * (sE[lc + EX] - sE[lc - EX]) + ((eu + ymE[fy + TX]) - (ypE[fy] + ed))) * rdx;
-  const T dvdx = d6f(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], rdx);
-  const T dudy = d6f(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX], sU[lc + 3 * EX], rdx);
+  T dFx, dEx, dFy, dEy, dvdx, dudy;
+  if constexpr (kWeno) {
+    const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
+    const T pc = sPt[lc];
+    const T cx = T(0.5) * (sFx[lc + 1] - sFx[lc - 1]) - T(0.5) * (lr * (sPt[lc + 1] - pc) - ll * (pc - sPt[lc - 1]));
+    const T cy = T(0.5) * (sFy[lc + EX] - sFy[lc - EX]) - T(0.5) * (lu * (sPt[lc + EX] - pc) - ld * (pc - sPt[lc - EX]));
+    dFx = (cx + ((fr + xmF[fx + 1]) - (xpF[fx] + fl))) * rdx;
+    dEx = (T(0.5) * (sE[lc + 1] - sE[lc - 1]) + ((er + xmE[fx + 1]) - (xpE[fx] + el))) * rdx;
+    dFy = (cy + ((fu + ymF[fy + TX]) - (ypF[fy] + fd))) * rdx;
+    dEy = (T(0.5) * (sE[lc + EX] - sE[lc - EX]) + ((eu + ymE[fy + TX]) - (ypE[fy] + ed))) * rdx;
+    dvdx = d6f(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], rdx);
+    dudy = d6f(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX], sU[lc + 3 * EX], rdx);
+  } else {
+    // centred differences of the cell values, stride 1 (x) or EX (y), of
+    // the same fields: the flux divergence, the gradient of E and the
+    // vorticity all take the operator's order
+    // (spatial_operators_mod_{2nd,4th,6th}.F90)
+    auto cd = [&](const T* f, int s) -> T {
+      if (OP == 1) return (f[lc + s] - f[lc - s]) * (T(0.5) * rdx);
+      if (OP == 2)
+        return ((f[lc - 2 * s] - f[lc + 2 * s]) + T(8) * (f[lc + s] - f[lc - s])) * (rdx * (T(1) / T(12)));
+      return d6f(f[lc - 3 * s], f[lc - 2 * s], f[lc - s], f[lc + s], f[lc + 2 * s], f[lc + 3 * s], rdx);
+    };
+    dFx = cd(sFx, 1);
+    dEx = cd(sE, 1);
+    dFy = cd(sFy, EX);
+    dEy = cd(sE, EX);
+    dvdx = cd(sV, 1);
+    dudy = cd(sU, EX);
+  }
   const unsigned go = (gt + __umul24(static_cast<unsigned>(ty + 3), P) + static_cast<unsigned>(tx + 3)) * W;
   const T zeta = dvdx - dudy + ldo(st.fs, go);
   const T tp = -(dFx + dFy);
@@ -507,14 +534,14 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   }
 }
 
-template <typename T, int KIND, int TY>
+template <typename T, int KIND, int TY, int OP>
 __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
   const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
   const unsigned tq = t / static_cast<unsigned>(a.nbx);
   const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
   const int i0 = static_cast<int>(t - tq * a.nbx) * TX;
   const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
-  stage_tile<T, KIND, TY>(a, st, hs.r[p], p, i0, j0, static_cast<int>(threadIdx.x));
+  stage_tile<T, KIND, TY, OP>(a, st, hs.r[p], p, i0, j0, static_cast<int>(threadIdx.x));
 }
 
 // ---- batched ensemble: every member's tiles in one launch per RK stage.
@@ -558,7 +585,7 @@ __device__ __forceinline__ void ens_tile(const EnsMember<T>* __restrict__ md, co
   const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
   const int i0 = static_cast<int>(lt - tq * a.nbx) * TX;
   const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
-  stage_tile<T, KIND, TY>(a, M->st, M->hs[p], p, i0, j0, tid);
+  stage_tile<T, KIND, TY, 0>(a, M->st, M->hs[p], p, i0, j0, tid);
 }
 
 // PERSIST = false: one workgroup per tile, in the XCD-aware order.
@@ -680,31 +707,42 @@ __global__ void __launch_bounds__(kDT) split_diag_final(const double* partials, 
   result[1] = E.s + (E.c + ce);
 }
 
-template <typename T, int TY>
-void launch_split_ty(hipStream_t s, int kind, const SplitArgs<T>& a, const SplitStatics<T>& st,
+template <typename T, int TY, int OP>
+void launch_split_op(hipStream_t s, int kind, const SplitArgs<T>& a, const SplitStatics<T>& st,
                      const HaloScatter& hs) {
   const dim3 grid(static_cast<unsigned>(a.nbx * a.nby * 6));
   const dim3 block(TX * TY);
   switch (kind) {
-    case kTendOnly: hipLaunchKernelGGL((stage_split<T, kTendOnly, TY>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s1: hipLaunchKernelGGL((stage_split<T, kRK3s1, TY>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s2: hipLaunchKernelGGL((stage_split<T, kRK3s2, TY>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s3: hipLaunchKernelGGL((stage_split<T, kRK3s3, TY>), grid, block, 0, s, a, st, hs); break;
-    case kRK4a: hipLaunchKernelGGL((stage_split<T, kRK4a, TY>), grid, block, 0, s, a, st, hs); break;
-    default: hipLaunchKernelGGL((stage_split<T, kRK4f, TY>), grid, block, 0, s, a, st, hs); break;
+    case kTendOnly: hipLaunchKernelGGL((stage_split<T, kTendOnly, TY, OP>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_split<T, kRK3s1, TY, OP>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_split<T, kRK3s2, TY, OP>), grid, block, 0, s, a, st, hs); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_split<T, kRK3s3, TY, OP>), grid, block, 0, s, a, st, hs); break;
+    case kRK4a: hipLaunchKernelGGL((stage_split<T, kRK4a, TY, OP>), grid, block, 0, s, a, st, hs); break;
+    default: hipLaunchKernelGGL((stage_split<T, kRK4f, TY, OP>), grid, block, 0, s, a, st, hs); break;
+  }
+}
+
+template <typename T, int TY>
+void launch_split_ty(hipStream_t s, int kind, int op, const SplitArgs<T>& a, const SplitStatics<T>& st,
+                     const HaloScatter& hs) {
+  switch (op) {
+    case CCS_OP_CENTRED2: launch_split_op<T, TY, 1>(s, kind, a, st, hs); break;
+    case CCS_OP_CENTRED4: launch_split_op<T, TY, 2>(s, kind, a, st, hs); break;
+    case CCS_OP_CENTRED6: launch_split_op<T, TY, 3>(s, kind, a, st, hs); break;
+    default: launch_split_op<T, TY, 0>(s, kind, a, st, hs); break;
   }
 }
 
 // Tile heights instantiated (8, 16, 32); the launch fills nbx, nby for the
-// chosen one.
+// chosen one.  op: CCS_OP_WENO or a centred CCS_OP_CENTRED{2,4,6}.
 template <typename T>
-void launch_split(hipStream_t s, int kind, SplitArgs<T> a, const SplitStatics<T>& st, const HaloScatter& hs,
-                  int tile_y) {
+void launch_split(hipStream_t s, int kind, int op, SplitArgs<T> a, const SplitStatics<T>& st,
+                  const HaloScatter& hs, int tile_y) {
   a.nbx = (a.n + TX - 1) / TX;
   a.nby = (a.n + tile_y - 1) / tile_y;
-  if (tile_y == 16) launch_split_ty<T, 16>(s, kind, a, st, hs);
-  else if (tile_y == 32) launch_split_ty<T, 32>(s, kind, a, st, hs);
-  else launch_split_ty<T, 8>(s, kind, a, st, hs);
+  if (tile_y == 16) launch_split_ty<T, 16>(s, kind, op, a, st, hs);
+  else if (tile_y == 32) launch_split_ty<T, 32>(s, kind, op, a, st, hs);
+  else launch_split_ty<T, 8>(s, kind, op, a, st, hs);
 }
 
 }  // namespace
@@ -770,7 +808,7 @@ void enqueue_step_t(SplitState* s, hipStream_t st, int scheme, hipEvent_t* ev) {
   int stage = 0;
   auto launch = [&](int kind) {
     if (ev) (void)hipEventRecord(ev[2 * stage], st);
-    launch_split<T>(st, kind, a, S, s->hs, s->tile_y);
+    launch_split<T>(st, kind, s->cfg.spatial_op, a, S, s->hs, s->tile_y);
     if (ev) (void)hipEventRecord(ev[2 * stage + 1], st);
     ++stage;
   };
@@ -1025,7 +1063,7 @@ int split_tendency(SplitState* s, hipStream_t st, bool, double* tp, double* tu, 
     SplitArgs<double> a = base_args<double>(s);
     a.in = cprog_of<double>(s, 0); a.base = a.in;
     for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<double*>(s->tend[k]);
-    launch_split<double>(st, kTendOnly, a, statics_of<double>(s), s->hs, s->tile_y);
+    launch_split<double>(st, kTendOnly, s->cfg.spatial_op, a, statics_of<double>(s), s->hs, s->tile_y);
     download_to_double<double>(st, s->tend[0], s->ni, t0);
     download_to_double<double>(st, s->tend[1], s->ni, t1);
     download_to_double<double>(st, s->tend[2], s->ni, t2);
@@ -1033,7 +1071,7 @@ int split_tendency(SplitState* s, hipStream_t st, bool, double* tp, double* tu, 
     SplitArgs<float> a = base_args<float>(s);
     a.in = cprog_of<float>(s, 0); a.base = a.in;
     for (int k = 0; k < 3; ++k) a.tend[k] = static_cast<float*>(s->tend[k]);
-    launch_split<float>(st, kTendOnly, a, statics_of<float>(s), s->hs, s->tile_y);
+    launch_split<float>(st, kTendOnly, s->cfg.spatial_op, a, statics_of<float>(s), s->hs, s->tile_y);
     download_to_double<float>(st, s->tend[0], s->ni, t0);
     download_to_double<float>(st, s->tend[1], s->ni, t1);
     download_to_double<float>(st, s->tend[2], s->ni, t2);

@@ -13,7 +13,7 @@
 ! operator: 'WENO' (spatial_operators_mod.F90, the default), '2nd', '4th',
 ! '6th' (spatial_operators_mod_2nd/4th/6th.F90) or 'WENO_ext'
 ! (spatial_operators_mod_WENO.F90, with mesh%matrixIG_ext from the mesh file's
-! jab or the generated map).  The variants run with precision = 'fp64'.
+! jab or the generated map).  'WENO_ext' runs with precision = 'fp64' only.
 !
 ! Differences, all in the optional group &gpu_opt that follows &dynamic_opt:
 !   nx         cells per panel edge.  When &domain mesh_file names an existing

@@ -66,7 +66,9 @@ enum ccs_eflux { CCS_EFLUX_NONE = 0 };
 /* Spatial operator: which of the reference's interchangeable
  * spatial_operators_mod sources the build compiles in (the namelist's
  * reconstruct_scheme, parameters_mod.F90:30, selects none of them; the module
- * file is swapped in the build).  Only CCS_FP64_PARITY runs the variants.
+ * file is swapped in the build).  The split precisions (CCS_FP32_SPLIT,
+ * CCS_FP64_FAST) run CCS_OP_WENO and the centred operators; CCS_OP_WENO_EXT
+ * runs on CCS_FP64_PARITY only.
  *  CCS_OP_WENO      spatial_operators_mod.F90: WENO-Z + Rusanov, face speed
  *                   from the mean of the two cells' G^-1, 6th-order vorticity
  *  CCS_OP_CENTRED2  spatial_operators_mod_2nd.F90: centred 2nd-order

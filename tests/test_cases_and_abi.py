@@ -101,8 +101,8 @@ def test_create_rejects_invalid_configs_before_touching_a_device():
     h = ctypes.c_void_p()
     for bad in (dict(n=3), dict(halo=2), dict(precision=9), dict(scheme=5), dict(dt=0.0), dict(eflux=1),
                 dict(spatial_op=5), dict(spatial_op=-1),
-                # the operator variants run on the FP64 parity path only
-                dict(spatial_op=1, precision=1), dict(spatial_op=4, precision=2)):
+                # the extended-lattice operator runs on the FP64 parity path only
+                dict(spatial_op=4, precision=1), dict(spatial_op=4, precision=2)):
         cfg = _lib.CcsConfig(n=32, halo=3, precision=0, scheme=0, eflux=0, device=0, use_graph=0, dt=60.0,
                              dx=0.05, radius=6.371e6, gravity=9.80616, omega=7.292e-5)
         for k, v in bad.items():

@@ -71,3 +71,55 @@ def test_weno_ext_from_mesh_file_metric(ccs, orc, tmp_path):
     mask = ~mesh.corner_mask().ravel()
     for k in ("phiG", "u", "v"):
         assert np.array_equal(getattr(s, k)[mask], o.a[k][mask])
+
+
+# ---- the centred operators on the split precisions
+# CCS_FP32_SPLIT and CCS_FP64_FAST run the centred operators with the split
+# path's arithmetic (geopotential as a deviation from a reference field,
+# contravariant winds U/sqrtg on the conformal grid), so they are close to the
+# oracle, not bitwise.  The fp64_fast bound is far below the distance between
+# two different operators, so it also pins which operator ran.
+CENTRED = ["centred2", "centred4", "centred6"]
+
+
+def _nl2(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64_fast", 1e-11), ("fp32", 2e-3)])
+@pytest.mark.parametrize("op", CENTRED)
+def test_split_centred_tendency_close_to_oracle(ccs, orc, op, precision, tol):
+    mesh, s0, dt = _setup(ccs, 32, 3)
+    print(f"split tendency C32 {precision} op={op}", flush=True)
+    with ccs.DynamicalCore(mesh, dt, precision=precision, spatial_op=op) as core:
+        core.set_state(s0)
+        tg = core.spatial_operator()
+    to = orc.Oracle(mesh, s0, dt, op=op).tendency()
+    tw = orc.Oracle(mesh, s0, dt, op="weno").tendency()
+    for name, a, b, w in zip(("phiG", "u", "v"), tg, to, tw):
+        e, ew = _nl2(a, b), _nl2(a, w)
+        print(f"  L({name}) normalised L2 diff {e:.3e} (against the WENO operator {ew:.3e})")
+        assert e <= tol
+        assert ew >= 1e-6
+
+
+@pytest.mark.parametrize("precision,scheme,nsteps,tol", [
+    ("fp64_fast", "RK3_TVD", 12, 1e-11), ("fp64_fast", "RK4", 5, 1e-11),
+    ("fp32", "RK3_TVD", 12, 1e-5), ("fp32", "RK4", 5, 1e-5)])
+@pytest.mark.parametrize("op", CENTRED)
+def test_split_centred_steps_close_to_parity_path(ccs, op, precision, scheme, nsteps, tol):
+    # against the FP64 parity path of the same operator, which equals the
+    # oracle bit for bit (test_variant_steps_bitwise)
+    mesh, s0, dt = _setup(ccs, 32, 3, case_num=7, mountain=True, seed=2)
+    print(f"{scheme} C32 {precision} op={op}: {nsteps} steps", flush=True)
+    states = {}
+    for prec in (precision, "fp64"):
+        with ccs.DynamicalCore(mesh, dt, precision=prec, scheme=scheme, spatial_op=op) as core:
+            core.set_state(s0)
+            core.step(nsteps)
+            states[prec] = core.get_state()
+    h = mesh.interior
+    for k in ("phi", "u", "v"):
+        e = _nl2(h(getattr(states[precision], k)), h(getattr(states["fp64"], k)))
+        print(f"  {k}: normalised L2 diff {e:.3e}")
+        assert e <= tol
