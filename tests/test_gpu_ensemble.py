@@ -72,6 +72,25 @@ def test_autotuned_members_equal_single_runs_at_the_chosen_tile(ccs, monkeypatch
             assert np.array_equal(getattr(got[k], f), getattr(ref, f)), (k, f)
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_members_at_the_benchmark_sizes_equal_single_runs(ccs, persistent):
+    # one member of each size class of the 64-member benchmark ensemble
+    # (N in {48, 96, 144, 192}), 21 steps: one 16-step graph, five single steps
+    sizes = (48, 96, 144, 192)
+    meshes, states, dts = _members(ccs, sizes=sizes)
+    print(f"fp32 RK3_TVD persistent={persistent}: members {sizes}, 21 steps", flush=True)
+    with ccs.Ensemble(meshes, dts, precision="fp32", tile_y=16, persistent=persistent) as ens:
+        for k, s in enumerate(states):
+            ens.set_state(k, s)
+        ens.step(21)
+        for k in range(len(ens)):
+            got = ens.get_state(k)
+            ref, (m1, e1) = _single(ccs, meshes[k], states[k], dts[k], "fp32", "RK3_TVD", 21)
+            for f in ("phiG", "u", "v"):
+                assert np.array_equal(getattr(got, f), getattr(ref, f)), (k, f)
+            assert ens.diagnostics(k) == (m1, e1)
+
+
 def test_member_mass_is_conserved(ccs):
     meshes, states, dts = _members(ccs, sizes=(24, 32), hours=24.0)
     with ccs.Ensemble(meshes, dts, precision="fp32", tile_y=16, persistent=1) as ens:
