@@ -104,7 +104,9 @@ struct SplitArgs {
   int nbx, nby;        // tiles per panel along x and y
   T rdx;               // 1 / dx
   T dts, incs;         // dt / a, RK4 increment / a
-  T epsF, epsE, epsP;  // WENO-Z epsilons of Fx/Fy, E, phitG/a^2
+  // WENO-Z epsilons of Fx/Fy, E (windows scaled by 1/2: times 1/16) and of
+  // phitG/a^2 (scaled by 1/4: times 1/64)
+  T epsF, epsE, epsP;
 };
 
 // One 5-point window (q0..q4, centre q2): the WENO-Z value at the centre
@@ -120,58 +122,66 @@ struct SplitArgs {
 //   left face : s1 = q2 + a3/12 + b3/4, s2 = q2 + a2/12 + b2/4, s3 = q2 + a1/12 - b1/4
 // and the branch weights (1/3,2/3,0), (0,2/3,1/3) are the unnormalised
 // weights (1,2,0), (0,2,1) fed through the same normalisation.
-template <typename T>
+//
+// Scaled window (SC = 2 or 4): the candidates, hence the returned face
+// values, are 1/SC of the true deviations, so a flux share 0.5 (F + lam P)
+// becomes one FMA of the scaled values.  The indicators come out multiplied
+// by (1/(2 SC))^2 and the caller passes epsilon multiplied by the same power
+// of two, so tau5 / (eps + beta_k), and with it every weight, is that of the
+// unscaled window.
+template <typename T, int SC>
 struct WenoWin {
-  T c1, c2, c3;     // a_k/12 (relative to q2)
-  T h1, h2, h3;     // b_k/2
-  T t40, t41, mb;   // indicator differences and min
-  T tr1, tr2, tr3;  // 1 + tau5 / (eps + beta_k)
+  T a1, a2, a3;     // second differences (relative to q2)
+  T q1, q2, q3;     // first differences b_k / (4 SC)
+  // WENO-Z weights 10 d_k (1 + tau5 / (eps + beta_k)), 10 d = (1, 6, 3): the
+  // right face takes (u1, u2, u3), the mirrored left face (v3, u2, v1), with
+  // v3 = u3 / 3 and v1 = 3 u1
+  T u1, u2, u3, v1, v3;
+  T t40, t41, mb;   // indicator differences and min (weight-rule branches)
 };
 
-// The window is evaluated relative to its centre value q2: the inputs are
-// the differences q_k - q2 (exact in FP32 for neighbouring values of one
-// sign, by Sterbenz' lemma) and the reconstructions are returned as
-// deviations from q2.  The weights only see differences, so they are those
-// of the absolute form; the absolute face value is q2 + deviation.
-template <typename T>
-__device__ __forceinline__ WenoWin<T> weno_window(T q0, T q1, T q2, T q3, T q4, T eps) {
+template <typename T, int SC>
+__device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T q4, T eps_scaled) {
 #pragma clang fp contract(fast)
-  WenoWin<T> w;
+  WenoWin<T, SC> w;
   const T d0 = q0 - q2, d1 = q1 - q2, d3 = q3 - q2, d4 = q4 - q2;
-  const T a1 = T(-2) * d1 + d0;
-  const T a2 = d1 + d3;
-  const T a3 = T(-2) * d3 + d4;
-  w.h1 = T(0.5) * a1 - d1;
-  w.h2 = T(0.5) * (d1 - d3);
-  w.h3 = T(0.5) * a3 - d3;
-  const T k13 = T(13) / T(12);
-  const T be1 = (k13 * a1) * a1 + w.h1 * w.h1;
-  const T be2 = (k13 * a2) * a2 + w.h2 * w.h2;
-  const T be3 = (k13 * a3) * a3 + w.h3 * w.h3;
+  w.a1 = T(-2) * d1 + d0;
+  w.a2 = d1 + d3;
+  w.a3 = T(-2) * d3 + d4;
+  // b1/2 = a1/2 - d1, b2/2 = (d1 - d3)/2, b3/2 = a3/2 - d3, each times 1/(2 SC)
+  constexpr T kh = T(1) / T(2 * SC), kq = T(1) / T(4 * SC);
+  w.q1 = kq * w.a1 - kh * d1;
+  w.q2 = kq * (d1 - d3);
+  w.q3 = kq * w.a3 - kh * d3;
+  // beta_k / (2 SC)^2 = 13/12 a_k^2 / (2 SC)^2 + (b_k / (4 SC))^2
+  const T k13 = (T(13) / T(12)) * (T(1) / T(4 * SC * SC));
+  const T be1 = (k13 * w.a1) * w.a1 + w.q1 * w.q1;
+  const T be2 = (k13 * w.a2) * w.a2 + w.q2 * w.q2;
+  const T be3 = (k13 * w.a3) * w.a3 + w.q3 * w.q3;
   w.t40 = fabs(be1 - be2);
   w.t41 = fabs(be2 - be3);
   const T t5 = fabs(be3 - be1);
   w.mb = fmin(be1, fmin(be2, be3));
-  w.tr1 = t5 * rcp(eps + be1) + T(1);
-  w.tr2 = t5 * rcp(eps + be2) + T(1);
-  w.tr3 = t5 * rcp(eps + be3) + T(1);
-  const T k12 = T(1) / T(12);
-  w.c1 = a1 * k12;
-  w.c2 = a2 * k12;
-  w.c3 = a3 * k12;
+  // (d1, d2, d3) = (0.1, 0.6, 0.3) times 10, which the normalisation removes;
+  // the factors 3 and 6 multiply, since fma cannot take them as an inline
+  // addend
+  const T r1 = rcp(eps_scaled + be1), r2 = rcp(eps_scaled + be2), r3 = rcp(eps_scaled + be3);
+  w.u1 = t5 * r1 + T(1);
+  w.v3 = t5 * r3 + T(1);
+  w.u2 = T(6) * (t5 * r2 + T(1));
+  w.u3 = T(3) * w.v3;
+  w.v1 = T(3) * w.u1;
   return w;
 }
 
-// WENO-Z combination of the three candidates with the linear weights
-// (d1, d2, d3) = (0.1, 0.6, 0.3) and u_k = 1 + tau5 / (eps + beta_k):
-// w_k = d_k u_k.  The branch cases of the weight rule (br1: (1/3, 2/3, 0),
+// WENO-Z combination of the three candidates with unnormalised weights
+// (w1, w2, w3).  The branch cases of the weight rule (br1: (1/3, 2/3, 0),
 // br2: (0, 2/3, 1/3)) replace the weights by (1, 2, 0) or (0, 2, 1) before the
 // same normalisation.  They occur only at local extrema of the indicators, so
 // the replacement sits behind a branch that whole wavefronts usually skip.
 template <typename T>
-__device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T u1, T u2, T u3, bool br1, bool br2) {
+__device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T w1, T w2, T w3, bool br1, bool br2) {
 #pragma clang fp contract(fast)
-  T w1 = T(0.1) * u1, w2 = T(0.6) * u2, w3 = T(0.3) * u3;
   if (br1 || br2) {
     // an empty volatile asm keeps this a branch (skipped by whole
     // wavefronts) instead of being if-converted into per-lane selects
@@ -183,38 +193,41 @@ __device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T u1, T u2, T u3, bo
   return (w1 * s1 + w2 * s2 + w3 * s3) * rcp(w1 + w2 + w3);
 }
 
-// value at the right face of the window's centre cell
-template <typename T>
-__device__ __forceinline__ T weno_right(const WenoWin<T>& w) {
+// value at the right face of the window's centre cell, times 1/SC
+template <typename T, int SC>
+__device__ __forceinline__ T weno_right(const WenoWin<T, SC>& w) {
 #pragma clang fp contract(fast)
-  const T s1 = w.c1 + T(0.5) * w.h1, s2 = w.c2 - T(0.5) * w.h2, s3 = w.c3 - T(0.5) * w.h3;
+  constexpr T kc = T(1) / T(12 * SC);
+  const T s1 = w.a1 * kc + w.q1, s2 = w.a2 * kc - w.q2, s3 = w.a3 * kc - w.q3;
   const bool b1 = (w.t40 <= w.mb) && (w.t41 > w.mb);
   const bool b2 = (w.t40 > w.mb) && (w.t41 < w.mb);
-  return weno_combine(s1, s2, s3, w.tr1, w.tr2, w.tr3, b1, b2);
+  return weno_combine(s1, s2, s3, w.u1, w.u2, w.u3, b1, b2);
 }
 
-// value at the left face of the window's centre cell (mirrored window)
-template <typename T>
-__device__ __forceinline__ T weno_left(const WenoWin<T>& w) {
+// value at the left face of the window's centre cell (mirrored window),
+// times 1/SC
+template <typename T, int SC>
+__device__ __forceinline__ T weno_left(const WenoWin<T, SC>& w) {
 #pragma clang fp contract(fast)
-  const T s1 = w.c3 + T(0.5) * w.h3, s2 = w.c2 + T(0.5) * w.h2, s3 = w.c1 - T(0.5) * w.h1;
+  constexpr T kc = T(1) / T(12 * SC);
+  const T s1 = w.a3 * kc + w.q3, s2 = w.a2 * kc + w.q2, s3 = w.a1 * kc - w.q1;
   const bool b1 = (w.t41 <= w.mb) && (w.t40 > w.mb);
   const bool b2 = (w.t41 > w.mb) && (w.t40 < w.mb);
-  return weno_combine(s1, s2, s3, w.tr3, w.tr2, w.tr1, b1, b2);
+  return weno_combine(s1, s2, s3, w.v3, w.u2, w.v1, b1, b2);
 }
 
-// either face, chosen per lane (left = true: the left face)
-template <typename T>
-__device__ __forceinline__ T weno_side(const WenoWin<T>& w, bool left) {
+// either face, chosen per lane (left = true: the left face), times 1/SC
+template <typename T, int SC>
+__device__ __forceinline__ T weno_side(const WenoWin<T, SC>& w, bool left) {
 #pragma clang fp contract(fast)
-  const T s1 = left ? w.c3 + T(0.5) * w.h3 : w.c1 + T(0.5) * w.h1;
-  const T s2 = w.c2 + (left ? T(0.5) : T(-0.5)) * w.h2;
-  const T s3 = left ? w.c1 - T(0.5) * w.h1 : w.c3 - T(0.5) * w.h3;
+  constexpr T kc = T(1) / T(12 * SC);
+  const T s1 = left ? w.a3 * kc + w.q3 : w.a1 * kc + w.q1;
+  const T s2 = w.a2 * kc + (left ? w.q2 : -w.q2);
+  const T s3 = left ? w.a1 * kc - w.q1 : w.a3 * kc - w.q3;
   const T ta = left ? w.t41 : w.t40, tb = left ? w.t40 : w.t41;
-  const T ra = left ? w.tr3 : w.tr1, rb = left ? w.tr1 : w.tr3;
   const bool b1 = (ta <= w.mb) && (tb > w.mb);
   const bool b2 = (ta > w.mb) && (tb < w.mb);
-  return weno_combine(s1, s2, s3, ra, w.tr2, rb, b1, b2);
+  return weno_combine(s1, s2, s3, left ? w.v3 : w.u1, w.u2, left ? w.v1 : w.u3, b1, b2);
 }
 
 template <typename T>
@@ -313,22 +326,21 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   }
   __syncthreads();
 
-  // Rusanov speed at the x-face between extended-tile cells l and l + 1, and
-  // at the y-face between l and l + EX (spatial_operators_mod.F90:109-116,
-  // 134-141, on a conformal grid)
+  // Twice the Rusanov speed at the x-face between extended-tile cells l and
+  // l + 1, and at the y-face between l and l + EX
+  // (spatial_operators_mod.F90:109-116, 134-141, on a conformal grid): the
+  // face means are left as sums, 2 |ub| + sqrt(4 gb pb) exactly.
   auto lam_x = [&](int l) {
     const T r0 = sRg[l], r1 = sRg[l + 1];
-    const T ub = T(0.5) * (sU[l] * r0 + sU[l + 1] * r1);
-    const T pb = T(0.5) * (sPt[l] * r0 + sPt[l + 1] * r1);
-    const T gb = T(0.5) * (r0 + r1);
-    return fabs(ub) + fsqrt(gb * pb);
+    const T ub = sU[l] * r0 + sU[l + 1] * r1;
+    const T pb = sPt[l] * r0 + sPt[l + 1] * r1;
+    return fabs(ub) + fsqrt((r0 + r1) * pb);
   };
   auto lam_y = [&](int l) {
     const T r0 = sRg[l], r1 = sRg[l + EX];
-    const T vb = T(0.5) * (sV[l] * r0 + sV[l + EX] * r1);
-    const T pb = T(0.5) * (sPt[l] * r0 + sPt[l + EX] * r1);
-    const T gb = T(0.5) * (r0 + r1);
-    return fabs(vb) + fsqrt(gb * pb);
+    const T vb = sV[l] * r0 + sV[l + EX] * r1;
+    const T pb = sPt[l] * r0 + sPt[l + EX] * r1;
+    return fabs(vb) + fsqrt((r0 + r1) * pb);
   };
 
   // ---- phase 2: WENO windows and face shares.  Thread (tx, ty) owns the
@@ -348,30 +360,31 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
     const int fx = ty * (TX + 1) + tx;   // left x-face of the cell
     const int fy = ty * TX + tx;         // bottom y-face of the cell
     {
-      const WenoWin<T> wp = weno_window(sPt[l - 2], sPt[l - 1], sPt[l], sPt[l + 1], sPt[l + 2], a.epsP);
-      const WenoWin<T> wf = weno_window(sFx[l - 2], sFx[l - 1], sFx[l], sFx[l + 1], sFx[l + 2], a.epsF);
-      const WenoWin<T> we = weno_window(sE[l - 2], sE[l - 1], sE[l], sE[l + 1], sE[l + 2], a.epsE);
+      const auto wp = weno_window<T, 4>(sPt[l - 2], sPt[l - 1], sPt[l], sPt[l + 1], sPt[l + 2], a.epsP);
+      const auto wf = weno_window<T, 2>(sFx[l - 2], sFx[l - 1], sFx[l], sFx[l + 1], sFx[l + 2], a.epsF);
+      const auto we = weno_window<T, 2>(sE[l - 2], sE[l - 1], sE[l], sE[l + 1], sE[l + 2], a.epsE);
       lr = lam_x(l);
       ll = lam_x(l - 1);
-      fr = T(0.5) * (weno_right(wf) + lr * weno_right(wp));
-      fl = T(0.5) * (weno_left(wf) - ll * weno_left(wp));
-      er = T(0.5) * weno_right(we);
-      el = T(0.5) * weno_left(we);
+      // (F + lam P)/2 = F/2 + (2 lam) (P/4)
+      fr = weno_right(wf) + lr * weno_right(wp);
+      fl = weno_left(wf) - ll * weno_left(wp);
+      er = weno_right(we);
+      el = weno_left(we);
       xpF[fx + 1] = fr;
       xmF[fx] = fl;
       xpE[fx + 1] = er;
       xmE[fx] = el;
     }
     {
-      const WenoWin<T> wp = weno_window(sPt[l - 2 * EX], sPt[l - EX], sPt[l], sPt[l + EX], sPt[l + 2 * EX], a.epsP);
-      const WenoWin<T> wf = weno_window(sFy[l - 2 * EX], sFy[l - EX], sFy[l], sFy[l + EX], sFy[l + 2 * EX], a.epsF);
-      const WenoWin<T> we = weno_window(sE[l - 2 * EX], sE[l - EX], sE[l], sE[l + EX], sE[l + 2 * EX], a.epsE);
+      const auto wp = weno_window<T, 4>(sPt[l - 2 * EX], sPt[l - EX], sPt[l], sPt[l + EX], sPt[l + 2 * EX], a.epsP);
+      const auto wf = weno_window<T, 2>(sFy[l - 2 * EX], sFy[l - EX], sFy[l], sFy[l + EX], sFy[l + 2 * EX], a.epsF);
+      const auto we = weno_window<T, 2>(sE[l - 2 * EX], sE[l - EX], sE[l], sE[l + EX], sE[l + 2 * EX], a.epsE);
       lu = lam_y(l);
       ld = lam_y(l - EX);
-      fu = T(0.5) * (weno_right(wf) + lu * weno_right(wp));
-      fd = T(0.5) * (weno_left(wf) - ld * weno_left(wp));
-      eu = T(0.5) * weno_right(we);
-      ed = T(0.5) * weno_left(we);
+      fu = weno_right(wf) + lu * weno_right(wp);
+      fd = weno_left(wf) - ld * weno_left(wp);
+      eu = weno_right(we);
+      ed = weno_left(we);
       ypF[fy + TX] = fu;
       ymF[fy] = fd;
       ypE[fy + TX] = eu;
@@ -411,16 +424,16 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
         lf = left ? l0 - EX : l0;  // cell below the face
       }
       const T* sF = (tid < NXE) ? sFx : sFy;
-      const WenoWin<T> wp = weno_window(sPt[l0 - 2 * stride], sPt[l0 - stride], sPt[l0], sPt[l0 + stride],
+      const auto wp = weno_window<T, 4>(sPt[l0 - 2 * stride], sPt[l0 - stride], sPt[l0], sPt[l0 + stride],
                                         sPt[l0 + 2 * stride], a.epsP);
-      const WenoWin<T> wf = weno_window(sF[l0 - 2 * stride], sF[l0 - stride], sF[l0], sF[l0 + stride],
+      const auto wf = weno_window<T, 2>(sF[l0 - 2 * stride], sF[l0 - stride], sF[l0], sF[l0 + stride],
                                         sF[l0 + 2 * stride], a.epsF);
-      const WenoWin<T> we = weno_window(sE[l0 - 2 * stride], sE[l0 - stride], sE[l0], sE[l0 + stride],
+      const auto we = weno_window<T, 2>(sE[l0 - 2 * stride], sE[l0 - stride], sE[l0], sE[l0 + stride],
                                         sE[l0 + 2 * stride], a.epsE);
-      const T lam = (tid < NXE) ? lam_x(lf) : lam_y(lf);
+      const T lam = (tid < NXE) ? lam_x(lf) : lam_y(lf);  // twice the speed
       const T vp = weno_side(wp, left), vf = weno_side(wf, left), ve = weno_side(we, left);
-      *oF = T(0.5) * (left ? vf - lam * vp : vf + lam * vp);
-      *oE = T(0.5) * ve;
+      *oF = left ? vf - lam * vp : vf + lam * vp;
+      *oE = ve;
     }
     __syncthreads();
   }
@@ -438,8 +451,9 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   if constexpr (kWeno) {
     const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
     const T pc = sPt[lc];
-    const T cx = T(0.5) * (sFx[lc + 1] - sFx[lc - 1]) - T(0.5) * (lr * (sPt[lc + 1] - pc) - ll * (pc - sPt[lc - 1]));
-    const T cy = T(0.5) * (sFy[lc + EX] - sFy[lc - EX]) - T(0.5) * (lu * (sPt[lc + EX] - pc) - ld * (pc - sPt[lc - EX]));
+    // (lr, ll, lu, ld hold twice the Rusanov speeds)
+    const T cx = T(0.5) * (sFx[lc + 1] - sFx[lc - 1]) - T(0.25) * (lr * (sPt[lc + 1] - pc) - ll * (pc - sPt[lc - 1]));
+    const T cy = T(0.5) * (sFy[lc + EX] - sFy[lc - EX]) - T(0.25) * (lu * (sPt[lc + EX] - pc) - ld * (pc - sPt[lc - EX]));
     dFx = (cx + ((fr + xmF[fx + 1]) - (xpF[fx] + fl))) * rdx;
     dEx = (T(0.5) * (sE[lc + 1] - sE[lc - 1]) + ((er + xmE[fx + 1]) - (xpE[fx] + el))) * rdx;
     dFy = (cy + ((fu + ymF[fy + TX]) - (ypF[fy] + fd))) * rdx;
@@ -795,9 +809,9 @@ SplitArgs<T> base_args(const SplitState* s) {
   a.h = s->h;
   a.rdx = static_cast<T>(1.0 / s->cfg.dx);
   a.dts = static_cast<T>(s->cfg.dt / r);
-  a.epsF = static_cast<T>(1e-2 / (r * r));
-  a.epsE = static_cast<T>(1e-2);
-  a.epsP = static_cast<T>(1e-2 / (r * r * r * r));
+  a.epsF = static_cast<T>(1e-2 / (r * r)) * T(1.0 / 16);
+  a.epsE = static_cast<T>(1e-2) * T(1.0 / 16);
+  a.epsP = static_cast<T>(1e-2 / (r * r * r * r)) * T(1.0 / 64);
   return a;
 }
 
