@@ -174,46 +174,49 @@ __device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T 
   return w;
 }
 
-// WENO-Z combination of the three candidates with unnormalised weights
-// (w1, w2, w3).  The branch cases of the weight rule (br1: (1/3, 2/3, 0),
-// br2: (0, 2/3, 1/3)) replace the weights by (1, 2, 0) or (0, 2, 1) before the
-// same normalisation.  They occur only at local extrema of the indicators, so
-// the replacement sits behind a branch that whole wavefronts usually skip.
+// The branch cases of the weight rule (spatial_operators_mod.F90:348-355):
+// the right face takes weights (1/3, 2/3, 0) when tau40 <= mb < tau41 and
+// (0, 2/3, 1/3) when tau41 < mb < tau40; the mirrored left face the same with
+// tau40 and tau41 swapped.  Each of the four cases has exactly one of
+// tau40 <= mb, tau41 <= mb, so a window with neither case leaves both tests
+// equal (also with NaN: every compare is then false).  They occur only at
+// local extrema of the indicators, so the exact cases sit behind that one
+// test, in a branch that whole wavefronts usually skip.  The branch weights
+// (1/3, 2/3, 0), (0, 2/3, 1/3) enter as the unnormalised (1, 2, 0), (0, 2, 1).
 template <typename T>
-__device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T w1, T w2, T w3, bool br1, bool br2) {
-#pragma clang fp contract(fast)
-  if (br1 || br2) {
-    // an empty volatile asm keeps this a branch (skipped by whole
-    // wavefronts) instead of being if-converted into per-lane selects
-    asm volatile("");
-    w1 = br1 ? T(1) : T(0);
+__device__ __forceinline__ void weno_branch_weights(T t40, T t41, T mb, T& w1, T& w2, T& w3) {
+  const bool b1 = (t40 <= mb) && (t41 > mb);
+  const bool b2 = (t40 > mb) && (t41 < mb);
+  if (b1 || b2) {
+    w1 = b1 ? T(1) : T(0);
     w2 = T(2);
-    w3 = br1 ? T(0) : T(1);
+    w3 = b1 ? T(0) : T(1);
   }
+}
+
+// normalised combination of the three candidates
+template <typename T>
+__device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T w1, T w2, T w3) {
+#pragma clang fp contract(fast)
   return (w1 * s1 + w2 * s2 + w3 * s3) * rcp(w1 + w2 + w3);
 }
 
-// value at the right face of the window's centre cell, times 1/SC
+// values at the right and the left face of the window's centre cell (the
+// left from the mirrored window), times 1/SC
 template <typename T, int SC>
-__device__ __forceinline__ T weno_right(const WenoWin<T, SC>& w) {
+__device__ __forceinline__ void weno_faces(const WenoWin<T, SC>& w, T& right, T& left) {
 #pragma clang fp contract(fast)
   constexpr T kc = T(1) / T(12 * SC);
-  const T s1 = w.a1 * kc + w.q1, s2 = w.a2 * kc - w.q2, s3 = w.a3 * kc - w.q3;
-  const bool b1 = (w.t40 <= w.mb) && (w.t41 > w.mb);
-  const bool b2 = (w.t40 > w.mb) && (w.t41 < w.mb);
-  return weno_combine(s1, s2, s3, w.u1, w.u2, w.u3, b1, b2);
-}
-
-// value at the left face of the window's centre cell (mirrored window),
-// times 1/SC
-template <typename T, int SC>
-__device__ __forceinline__ T weno_left(const WenoWin<T, SC>& w) {
-#pragma clang fp contract(fast)
-  constexpr T kc = T(1) / T(12 * SC);
-  const T s1 = w.a3 * kc + w.q3, s2 = w.a2 * kc + w.q2, s3 = w.a1 * kc - w.q1;
-  const bool b1 = (w.t41 <= w.mb) && (w.t40 > w.mb);
-  const bool b2 = (w.t41 > w.mb) && (w.t40 < w.mb);
-  return weno_combine(s1, s2, s3, w.v3, w.u2, w.v1, b1, b2);
+  T r1 = w.u1, r2 = w.u2, r3 = w.u3, l1 = w.v3, l2 = w.u2, l3 = w.v1;
+  if ((w.t40 <= w.mb) != (w.t41 <= w.mb)) {
+    // an empty volatile asm keeps this a branch (skipped by whole
+    // wavefronts) instead of being if-converted into per-lane selects
+    asm volatile("");
+    weno_branch_weights(w.t40, w.t41, w.mb, r1, r2, r3);
+    weno_branch_weights(w.t41, w.t40, w.mb, l1, l2, l3);
+  }
+  right = weno_combine(w.a1 * kc + w.q1, w.a2 * kc - w.q2, w.a3 * kc - w.q3, r1, r2, r3);
+  left = weno_combine(w.a3 * kc + w.q3, w.a2 * kc + w.q2, w.a1 * kc - w.q1, l1, l2, l3);
 }
 
 // either face, chosen per lane (left = true: the left face), times 1/SC
@@ -224,10 +227,12 @@ __device__ __forceinline__ T weno_side(const WenoWin<T, SC>& w, bool left) {
   const T s1 = left ? w.a3 * kc + w.q3 : w.a1 * kc + w.q1;
   const T s2 = w.a2 * kc + (left ? w.q2 : -w.q2);
   const T s3 = left ? w.a1 * kc - w.q1 : w.a3 * kc - w.q3;
-  const T ta = left ? w.t41 : w.t40, tb = left ? w.t40 : w.t41;
-  const bool b1 = (ta <= w.mb) && (tb > w.mb);
-  const bool b2 = (ta > w.mb) && (tb < w.mb);
-  return weno_combine(s1, s2, s3, left ? w.v3 : w.u1, w.u2, left ? w.v1 : w.u3, b1, b2);
+  T w1 = left ? w.v3 : w.u1, w2 = w.u2, w3 = left ? w.v1 : w.u3;
+  if ((w.t40 <= w.mb) != (w.t41 <= w.mb)) {
+    asm volatile("");
+    weno_branch_weights(left ? w.t41 : w.t40, left ? w.t40 : w.t41, w.mb, w1, w2, w3);
+  }
+  return weno_combine(s1, s2, s3, w1, w2, w3);
 }
 
 template <typename T>
@@ -365,11 +370,13 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       const auto we = weno_window<T, 2>(sE[l - 2], sE[l - 1], sE[l], sE[l + 1], sE[l + 2], a.epsE);
       lr = lam_x(l);
       ll = lam_x(l - 1);
+      T fR, fL, pR, pL;
+      weno_faces(wf, fR, fL);
+      weno_faces(wp, pR, pL);
+      weno_faces(we, er, el);
       // (F + lam P)/2 = F/2 + (2 lam) (P/4)
-      fr = weno_right(wf) + lr * weno_right(wp);
-      fl = weno_left(wf) - ll * weno_left(wp);
-      er = weno_right(we);
-      el = weno_left(we);
+      fr = fR + lr * pR;
+      fl = fL - ll * pL;
       xpF[fx + 1] = fr;
       xmF[fx] = fl;
       xpE[fx + 1] = er;
@@ -381,10 +388,12 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       const auto we = weno_window<T, 2>(sE[l - 2 * EX], sE[l - EX], sE[l], sE[l + EX], sE[l + 2 * EX], a.epsE);
       lu = lam_y(l);
       ld = lam_y(l - EX);
-      fu = weno_right(wf) + lu * weno_right(wp);
-      fd = weno_left(wf) - ld * weno_left(wp);
-      eu = weno_right(we);
-      ed = weno_left(we);
+      T fR, fL, pR, pL;
+      weno_faces(wf, fR, fL);
+      weno_faces(wp, pR, pL);
+      weno_faces(we, eu, ed);
+      fu = fR + lu * pR;
+      fd = fL - ld * pL;
       ypF[fy + TX] = fu;
       ymF[fy] = fd;
       ypE[fy + TX] = eu;
