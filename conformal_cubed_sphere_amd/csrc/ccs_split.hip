@@ -177,10 +177,10 @@ __device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T 
 // The branch cases of the weight rule (spatial_operators_mod.F90:348-355):
 // the right face takes weights (1/3, 2/3, 0) when tau40 <= mb < tau41 and
 // (0, 2/3, 1/3) when tau41 < mb < tau40; the mirrored left face the same with
-// tau40 and tau41 swapped.  Each of the four cases has exactly one of
-// tau40 <= mb, tau41 <= mb, so a window with neither case leaves both tests
-// equal (also with NaN: every compare is then false).  They occur only at
-// local extrema of the indicators, so the exact cases sit behind that one
+// tau40 and tau41 swapped.  In each of the four cases exactly one of
+// tau40 <= mb, tau41 <= mb holds, so where both tests agree no case applies
+// (also with NaN, every compare then being false).  The cases occur only at
+// local extrema of the indicators, so the exact tests sit behind that one
 // test, in a branch that whole wavefronts usually skip.  The branch weights
 // (1/3, 2/3, 0), (0, 2/3, 1/3) enter as the unnormalised (1, 2, 0), (0, 2, 1).
 template <typename T>
