@@ -1,6 +1,6 @@
 """Time the reference's own time step on the CPU (the cpu_baseline of bench.py).
 
-TEST/BASELINE INFRASTRUCTURE ONLY.  Loads oracle/_ref/libccs_ref_omp.so: the
+TEST AND CPU-BASELINE INFRASTRUCTURE ONLY.  Loads oracle/_ref/libccs_ref_omp.so: the
 reference's dynamic-core modules compiled unmodified from their sources with
 the reference's own GNU build flags (-O3 -fdefault-real-8 -fopenmp, see
 oracle/Makefile), driven by oracle/ref_harness.F90's RK3_TVD loop
