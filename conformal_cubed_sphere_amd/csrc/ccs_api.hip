@@ -622,9 +622,7 @@ extern "C" int ccs_ensemble_create(const ccs_ensemble_config* cfg, const int* n,
   *out = nullptr;
   if (cfg->n_members < 1 || cfg->halo < 3) return CCS_ERR_INVALID;
   if (cfg->precision != CCS_FP32_SPLIT && cfg->precision != CCS_FP64_FAST) return CCS_ERR_INVALID;
-  // RK4 ensembles are switched off: their first GPU run faulted and the
-  // cause is not yet found (the single-run RK4 path is unaffected)
-  if (cfg->scheme != CCS_RK3_TVD) return CCS_ERR_INVALID;
+  if (cfg->scheme != CCS_RK3_TVD && cfg->scheme != CCS_RK4) return CCS_ERR_INVALID;
   if (cfg->tile_y != 0 && cfg->tile_y != 8 && cfg->tile_y != 16 && cfg->tile_y != 32) return CCS_ERR_INVALID;
   if (cfg->persistent < -1 || cfg->persistent > 1) return CCS_ERR_INVALID;
   for (int k = 0; k < cfg->n_members; ++k)

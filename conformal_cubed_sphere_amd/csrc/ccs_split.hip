@@ -905,10 +905,14 @@ int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out,
     }
     s->tile_y = v;
   }
-  // typed: 4 statics + 9 state (padded) + 9 tendencies (interior)
-  const std::size_t typed = (4 + 9) * s->np + 9 * s->ni;
+  // One allocation: the FP64 block (Pref, sqrtG, phis, diagnostics partials),
+  // then 4 statics + 9 state arrays (padded) and 9 tendencies (interior).
+  // Every array starts on a 256-byte boundary, so the size is the sum of the
+  // rounded sizes, and the carving below walks the same list.
+  auto rup = [](std::size_t b) { return (b + 255) / 256 * 256; };
   const std::size_t dbl = 3 * s->np + 4 * kSplitDiagBlocks + 2;
-  const std::size_t bytes = typed * s->esz + dbl * sizeof(double) + 256;
+  const std::size_t db = rup(dbl * sizeof(double)), fb = rup(s->np * s->esz), tb = rup(s->ni * s->esz);
+  const std::size_t bytes = db + (4 + 9) * fb + 9 * tb;
   if (hipMalloc(&s->dmem, bytes) != hipSuccess) {
     *err = "split_create: device allocation failed";
     delete s;
@@ -920,15 +924,22 @@ int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out,
     delete s;
     return CCS_ERR_DEVICE;
   }
-  char* p = static_cast<char*>(s->dmem);
-  auto take = [&p](std::size_t b) { void* r = p; p += (b + 255) / 256 * 256; return r; };
-  double* d = reinterpret_cast<double*>(take(dbl * sizeof(double)));
+  char* const base = static_cast<char*>(s->dmem);
+  char* p = base;
+  auto take = [&p](std::size_t rounded) { void* r = p; p += rounded; return r; };
+  double* d = reinterpret_cast<double*>(take(db));
   s->pref64_d = d; s->sqrtG_d = d + s->np; s->phis_d = d + 2 * s->np;
   s->partials = d + 3 * s->np; s->result = s->partials + 4 * kSplitDiagBlocks;
-  s->pref = take(s->np * s->esz); s->sg = take(s->np * s->esz); s->phis = take(s->np * s->esz); s->fs = take(s->np * s->esz);
+  s->pref = take(fb); s->sg = take(fb); s->phis = take(fb); s->fs = take(fb);
   for (int b = 0; b < 3; ++b)
-    for (int k = 0; k < 3; ++k) s->buf[b][k] = take(s->np * s->esz);
-  for (int k = 0; k < 9; ++k) s->tend[k] = take(s->ni * s->esz);
+    for (int k = 0; k < 3; ++k) s->buf[b][k] = take(fb);
+  for (int k = 0; k < 9; ++k) s->tend[k] = take(tb);
+  if (static_cast<std::size_t>(p - base) != bytes) {  // the carving and the size disagree
+    *err = "split_create: internal error in the device buffer layout";
+    (void)hipFree(s->dmem);
+    delete s;
+    return CCS_ERR_INVALID;
+  }
   *out = s;
   return CCS_OK;
 }

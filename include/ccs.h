@@ -287,7 +287,7 @@ typedef struct ccs_ensemble_config {
   int n_members;
   int halo;              /* >= 3, shared by all members                      */
   int precision;         /* CCS_FP32_SPLIT or CCS_FP64_FAST                   */
-  int scheme;            /* CCS_RK3_TVD (RK4 is rejected for ensembles)      */
+  int scheme;            /* CCS_RK3_TVD or CCS_RK4                            */
   int device;
   int use_graph;         /* replay steps from captured HIP graphs             */
   int tile_y;            /* 8, 16, 32; 0 = chosen by the autotuner            */

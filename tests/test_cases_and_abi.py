@@ -139,7 +139,7 @@ def test_ensemble_create_rejects_invalid_configs_before_touching_a_device():
     n = (ctypes.c_int * 2)(32, 48)
     dt = np.array([60.0, 40.0])
     dx = np.array([0.05, 0.03])
-    for bad in (dict(n_members=0), dict(halo=2), dict(precision=0), dict(scheme=3), dict(scheme=1), dict(tile_y=12),
+    for bad in (dict(n_members=0), dict(halo=2), dict(precision=0), dict(scheme=3), dict(scheme=-1), dict(tile_y=12),
                 dict(persistent=2)):
         cfg = _lib.CcsEnsembleConfig(n_members=2, halo=3, precision=1, scheme=0, device=0, use_graph=1, tile_y=0,
                                      persistent=-1, radius=6.371e6, gravity=9.80616, omega=7.292e-5)
@@ -155,17 +155,6 @@ def test_ensemble_create_rejects_invalid_configs_before_touching_a_device():
     assert _lib.lib.ccs_ensemble_step(None, 1) == _lib.CCS_ERR_INVALID
     assert _lib.lib.ccs_ensemble_last_error(None) == b"null ensemble"
     _lib.lib.ccs_ensemble_destroy(None)
-
-
-def test_ensemble_rejects_rk4_until_its_gpu_fault_is_explained():
-    from conformal_cubed_sphere_amd import _lib
-
-    h = ctypes.c_void_p()
-    n = (ctypes.c_int * 1)(32)
-    cfg = _lib.CcsEnsembleConfig(n_members=1, halo=3, precision=1, scheme=1, device=0, use_graph=1, tile_y=16,
-                                 persistent=0, radius=6.371e6, gravity=9.80616, omega=7.292e-5)
-    assert _lib.lib.ccs_ensemble_create(ctypes.byref(cfg), n, _lib.dptr(np.array([60.0])),
-                                        _lib.dptr(np.array([0.05])), ctypes.byref(h)) == _lib.CCS_ERR_INVALID
 
 
 def test_set_mesh_ext_rejects_null_and_handles():

@@ -31,9 +31,7 @@ def _single(ccs, mesh, s0, dt, precision, scheme, nsteps):
 
 @pytest.mark.parametrize("precision,scheme,persistent", [
     ("fp32", "RK3_TVD", 0), ("fp32", "RK3_TVD", 1),
-    pytest.param("fp32", "RK4", 1, marks=pytest.mark.skip(
-        reason="RK4 ensembles are switched off in ccs_ensemble_create: the first GPU run of this case faulted "
-               "and its cause is not found yet")),
+    ("fp32", "RK4", 1), ("fp32", "RK4", 0),
     ("fp64_fast", "RK3_TVD", 0), ("fp64_fast", "RK3_TVD", 1)])
 def test_members_equal_single_runs(ccs, precision, scheme, persistent):
     assert "CCS_SPLIT_TILE_Y" not in os.environ
