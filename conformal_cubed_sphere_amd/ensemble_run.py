@@ -43,14 +43,15 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64_fast"])
+    ap.add_argument("--scheme", default="RK3_TVD", choices=["RK3_TVD", "RK4"])
     ap.add_argument("--tile-y", type=int, default=0, help="8, 16, 32 or 0 (autotune)")
     ap.add_argument("--persistent", type=int, default=-1, help="0, 1 or -1 (autotune)")
     ap.add_argument("--sizes", default="48,96,144,192", help="panel sizes the members draw from")
     args = ap.parse_args()
     runs, ns = build_members(args.members, sizes=tuple(int(x) for x in args.sizes.split(",")))
     cells = sum(6 * n * n for n in ns)
-    with Ensemble([r[0] for r in runs], [r[2] for r in runs], precision=args.precision, tile_y=args.tile_y,
-                  persistent=args.persistent) as ens:
+    with Ensemble([r[0] for r in runs], [r[2] for r in runs], precision=args.precision, scheme=args.scheme,
+                  tile_y=args.tile_y, persistent=args.persistent) as ens:
         for k, r in enumerate(runs):
             ens.set_state(k, r[1])
         m0 = [ens.diagnostics(k)[0] for k in range(len(ens))]
@@ -64,7 +65,8 @@ def main():
         drift = [abs(ens.diagnostics(k)[0] - m0[k]) / m0[k] for k in range(len(ens))]
     print(json.dumps({"members": args.members, "interior_cells": cells, "steps": args.steps,
                       "member_cell_updates_per_s": cells * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
-                      "autotune": choice, "precision": args.precision, "mass_drift_max": max(drift)}), flush=True)
+                      "autotune": choice, "precision": args.precision, "scheme": args.scheme,
+                      "mass_drift_max": max(drift)}), flush=True)
 
 
 if __name__ == "__main__":
