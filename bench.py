@@ -257,6 +257,15 @@ def main():
             t = json.load(f)
         out["roofline"]["traffic"] = t.get("hbm_bytes_per_launch")
         out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
+    # the stage kernel is limited by VALU issue and latency, not by HBM: the
+    # committed SQ counters of the same kernel say how much
+    valu = os.path.join(ROOT, "profiles", f"valu_c{args.n}_{headline}.json")
+    if os.path.exists(valu) and args.spatial_op == "weno":
+        with open(valu) as f:
+            v = json.load(f)
+        out["roofline"]["valu"] = {"insts_per_wave": v.get("valu_insts_per_wave"),
+                                   "wait_inst_any_frac": v.get("wait_inst_any_frac_of_wave_cycles"),
+                                   "source": os.path.relpath(valu, ROOT)}
 
     if headline == "fp32" and not args.no_fp64:
         steps64 = max(20, args.steps // 3)
