@@ -32,7 +32,7 @@ def _single(ccs, mesh, s0, dt, precision, scheme, nsteps):
 @pytest.mark.parametrize("precision,scheme,persistent", [
     ("fp32", "RK3_TVD", 0), ("fp32", "RK3_TVD", 1),
     ("fp32", "RK4", 1), ("fp32", "RK4", 0),
-    ("fp64_fast", "RK3_TVD", 0), ("fp64_fast", "RK3_TVD", 1)])
+    ("fp64_fast", "RK3_TVD", 0), ("fp64_fast", "RK3_TVD", 1), ("fp64_fast", "RK4", 1)])
 def test_members_equal_single_runs(ccs, precision, scheme, persistent):
     assert "CCS_SPLIT_TILE_Y" not in os.environ
     meshes, states, dts = _members(ccs)
