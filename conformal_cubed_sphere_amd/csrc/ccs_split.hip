@@ -140,6 +140,11 @@ struct WenoWin {
   T t40, t41, mb;   // indicator differences and min (weight-rule branches)
 };
 
+// The window is evaluated relative to its centre value q2: the inputs are
+// the differences q_k - q2 (exact in FP32 for neighbouring values of one
+// sign, by Sterbenz' lemma) and the reconstructions are returned as
+// deviations from q2.  The weights only see differences, so they are those
+// of the absolute form; the absolute face value is q2 + deviation.
 template <typename T, int SC>
 __device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T q4, T eps_scaled) {
 #pragma clang fp contract(fast)
