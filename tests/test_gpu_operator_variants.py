@@ -123,3 +123,22 @@ def test_split_centred_steps_close_to_parity_path(ccs, op, precision, scheme, ns
         e = _nl2(h(getattr(states[precision], k)), h(getattr(states["fp64"], k)))
         print(f"  {k}: normalised L2 diff {e:.3e}")
         assert e <= tol
+
+
+@pytest.mark.parametrize("op", ["weno"] + OPS)
+def test_variant_steps_bitwise_at_c96(ccs, orc, op):
+    # a larger grid than the C32 cases above (3 tiles per panel row, ragged
+    # last tile column), the benchmark recipe with the mountain, 3 RK3 steps
+    mesh = ccs.Mesh(96, halo=3, radius=6.371e6, omega=7.292e-5)
+    s0 = ccs.init_case(mesh, case_num=7, seed=1, n_bumps=8, mountain=True)
+    dt, _ = ccs.cfl_time_step(mesh, s0, 0.5, 86400.0)
+    print(f"RK3 C96 op={op}: 3 steps", flush=True)
+    with ccs.DynamicalCore(mesh, dt, spatial_op=op) as core:
+        core.set_state(s0)
+        core.step(3)
+        s = core.get_state()
+    o = orc.Oracle(mesh, s0, dt, op=op)
+    o.steps(3)
+    mask = ~mesh.corner_mask().ravel()
+    for k in ("phiG", "u", "v", "phi", "uc", "vc"):
+        assert np.array_equal(getattr(s, k)[mask], o.a[k][mask]), k
