@@ -24,21 +24,32 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s at C720, FP32 vs FP64 GPU at matched L2(h) error; HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic words moved per interior cell by each SSP-RK3 stage kernel
-# (stages 1, 2, 3).  Split path: reads the stage input (dP, U, V) and the
-# statics {Pref, sqrtg, phis, a sqrtg f}, writes (dP, U, V); stages 2 and 3
-# also read the base state q^n (stage 1's base is its input).  FP64 parity
-# path: input (3) + statics {sqrtG, G^-1 (4), phis, f, A G^-1 (4),
-# G A^-1 (4)} + write (3), + base (3) in stages 2 and 3.
-ALG_WORDS_PER_STAGE = {"split": (10, 13, 13), "parity": (21, 24, 24)}
-WORD = {"fp32": 4, "fp64_fast": 8, "fp64": 8}
-
-
-def alg_bytes_per_launch(precision):
-    """Mean algorithmic HBM bytes per interior cell per stage-kernel launch."""
-    w = ALG_WORDS_PER_STAGE["parity" if precision == "fp64" else "split"]
-    return WORD[precision] * sum(w) / len(w)
 GRAVITY = 9.80616
+WORD = {"fp32": 4, "fp64_fast": 8, "fp64": 8}
+# Algorithmic bytes of one cell-update (one interior cell, one SSP-RK3 step),
+# the unit of the roofline figure: 24 state words (3 stages x (3 read + 3
+# written) + 2 stages x 3 base-state reads) and 9 static words (3 stages x
+# {sqrtG, f, phis}; G^-1 = I / sqrtG on the conformal grid): 33 words, 132 B
+# in FP32, 264 B in FP64.  Per stage launch: a third of that.
+UPDATE_WORDS = 33
+# What each stage kernel reads and writes besides (mean words per cell per
+# launch).  Split path: stage input (dP, U, V) + statics {Pref, sqrtg, phis,
+# a sqrtg f} + output (3), + base state in stages 2 and 3: (10 + 13 + 13) / 3
+# = 12 (the reference field Pref of the FP32 deviation is the word beyond the
+# 11 of the cell-update count).  FP64 parity path: input (3) + statics
+# {sqrtG, G^-1 (4), phis, f, A G^-1 (4), G A^-1 (4)} + output (3), + base (3)
+# in stages 2 and 3: (21 + 24 + 24) / 3 = 23.
+KERNEL_WORDS_PER_LAUNCH = {"split": 12.0, "parity": 23.0}
+
+
+def update_bytes_per_launch(precision):
+    """Algorithmic bytes per interior cell per stage launch (cell-update bytes / 3)."""
+    return WORD[precision] * UPDATE_WORDS / 3.0
+
+
+def kernel_bytes_per_launch(precision):
+    """Bytes per interior cell per stage launch that the kernel as built moves."""
+    return WORD[precision] * KERNEL_WORDS_PER_LAUNCH["parity" if precision == "fp64" else "split"]
 
 
 def parse():
@@ -57,7 +68,26 @@ def parse():
     ap.add_argument("--spatial-op", default="weno",
                     choices=["weno", "centred2", "centred4", "centred6", "weno_ext"],
                     help="spatial_operators_mod variant (weno_ext runs on the FP64 parity path only)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (--reps times), the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to HBM bytes, and exit")
+    ap.add_argument("--reps", type=int, default=5, help="copies of --copy-calibration")
     return ap.parse_args()
+
+
+def copy_calibration(mib, reps):
+    """A device-to-device copy of exactly mib MiB read and mib MiB written per
+    dispatch (torch as device-memory plumbing); under rocprofv3 --pmc the
+    counters it reads give the scale from counter units to HBM bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB x {reps}: read {mib} MiB and wrote {mib} MiB per dispatch", flush=True)
 
 
 def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno"):
@@ -174,18 +204,14 @@ def ensemble_bench(ccs, members, steps, warmup):
     # error against FP64, beside the method's own floor: an FP64 run from the
     # initial state perturbed at FP32 rounding level (2^-24 relative)
     l2, floor, drift = [], [], []
-    prng = np.random.default_rng(2024)
     for k, r in enumerate(runs):
         got = ens.get_state(k)
-        sp = r[1].copy()
-        for f in ("phiG", "u", "v"):
-            a = getattr(sp, f)
-            a *= 1.0 + 2.0 ** -24 * prng.standard_normal(a.shape)
+        sp = ccs.perturb_state(r[0], r[1], 2.0 ** -24)
         fin = []
-        for st0, comp in ((r[1], True), (sp, False)):
+        for st0, first in ((r[1], True), (sp, False)):
             with ccs.DynamicalCore(r[0], r[2], precision="fp64") as c:
-                c.set_state(st0, with_companions=comp)
-                if comp:
+                c.set_state(st0)
+                if first:
                     m0, _ = c.diagnostics()
                 c.step(warmup + steps)
                 fin.append(c.get_state())
@@ -209,6 +235,9 @@ def ensemble_bench(ccs, members, steps, warmup):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, args.reps)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
               "the multi-GPU panel sharding is a design only, not a runnable path", file=sys.stderr)
@@ -233,7 +262,8 @@ def main():
     core, wall, ev_total, kern_ms = time_path(ccs, mesh, s0, dt, headline, args.steps, args.warmup,
                                               args.spatial_op)
     value = cells * args.steps / wall
-    bpl = alg_bytes_per_launch(headline)
+    bpl = update_bytes_per_launch(headline)
+    kbpl = kernel_bytes_per_launch(headline)
     achieved = bpl * cells / (kern_ms * 1e-3) / 1e9  # GB/s of the stage kernel
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -248,7 +278,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "stage kernel (fused tendency + RK stage + halo exchange)",
                      "kernel_ms_avg": kern_ms, "algorithmic_bytes_per_launch": bpl * cells,
-                     "algorithmic_bytes_per_cell_per_launch": bpl},
+                     "algorithmic_bytes_per_cell_per_launch": bpl,
+                     "algorithmic_bytes_per_cell_update": 3 * bpl,
+                     "kernel_bytes_per_cell_per_launch": kbpl,
+                     "achieved_at_kernel_bytes": kbpl * cells / (kern_ms * 1e-3) / 1e9,
+                     "frac_at_kernel_bytes": kbpl * cells / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                     "limiter": "hbm" if args.spatial_op.startswith("centred") else "valu",
+                     "steps_per_s_over_stage_kernels": (3 * kern_ms) / (wall * 1e3 / args.steps)},
         "event_ms_per_step": ev_total / args.steps,
     }
     prof = os.path.join(ROOT, "profiles", f"traffic_c{args.n}_{headline}.json")
@@ -272,7 +308,8 @@ def main():
         c64, wall64, _, kern64 = time_path(ccs, mesh, s0, dt, "fp64", steps64, 5, args.spatial_op)
         v64 = cells * steps64 / wall64
         out["fp64_parity_path"] = {"value": v64, "unit": "cell-updates/s", "kernel_ms_avg": kern64,
-                                   "hbm_gbps_algorithmic": alg_bytes_per_launch("fp64") * cells / (kern64 * 1e-3) / 1e9}
+                                   "hbm_gbps_algorithmic": update_bytes_per_launch("fp64") * cells / (kern64 * 1e-3) / 1e9,
+                                   "hbm_gbps_kernel_bytes": kernel_bytes_per_launch("fp64") * cells / (kern64 * 1e-3) / 1e9}
         c64.close()
         cf, wallf, _, kernf = time_path(ccs, mesh, s0, dt, "fp64_fast", steps64, 5, args.spatial_op)
         out["fp64_split_kernel"] = {"value": cells * steps64 / wallf, "unit": "cell-updates/s", "kernel_ms_avg": kernf}
@@ -280,21 +317,15 @@ def main():
         out["fp32_over_fp64"] = value / v64
         # Matched error: FP32 vs FP64 after the same steps from the same start,
         # beside the method's own floor (an FP64 run whose initial state is
-        # perturbed at FP32 rounding level, 2^-24 relative).
-        import numpy as np
-
+        # perturbed at FP32 rounding level, 2^-24 relative, halo-consistent).
         s32 = core.get_state()
         core.close()
         nadv = core.advanced
-        rng = np.random.default_rng(2024)
-        sp = s0.copy()
-        for k in ("phiG", "u", "v"):
-            a = getattr(sp, k)
-            a *= 1.0 + 2.0 ** -24 * rng.standard_normal(a.shape)
+        sp = ccs.perturb_state(mesh, s0, 2.0 ** -24)
         states = {}
-        for name, st0, comp in (("ref", s0, True), ("pert", sp, False)):
+        for name, st0 in (("ref", s0), ("pert", sp)):
             c = ccs.DynamicalCore(mesh, dt, precision="fp64", spatial_op=args.spatial_op)
-            c.set_state(st0, with_companions=comp)
+            c.set_state(st0)
             c.step(nadv)
             states[name] = c.get_state()
             c.close()

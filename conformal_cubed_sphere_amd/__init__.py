@@ -18,6 +18,7 @@ from .core import (  # noqa: F401
     cfl_time_step,
     error_norms,
     init_case,
+    perturb_state,
     SPATIAL_OPS,
     read_mesh_file,
     read_mesh_file_ext,
@@ -25,4 +26,4 @@ from .core import (  # noqa: F401
 )
 
 __all__ = ["CcsError", "DynamicalCore", "Ensemble", "History", "Mesh", "SPATIAL_OPS", "State", "cfl_time_step",
-           "error_norms", "init_case", "read_mesh_file", "read_mesh_file_ext", "write_mesh_file"]
+           "error_norms", "init_case", "perturb_state", "read_mesh_file", "read_mesh_file_ext", "write_mesh_file"]

@@ -122,6 +122,10 @@ SIGNATURES = {
     "ccs_ensemble_get_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D]),
     "ccs_ensemble_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D]),
     "ccs_ensemble_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ccs_ensemble_step_members": (ctypes.c_int, [ctypes.c_void_p, _I]),
+    "ccs_ensemble_run_until": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "ccs_ensemble_steps_done": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]),
+    "ccs_ensemble_autotune_source": (ctypes.c_int, [ctypes.c_void_p, _I]),
     "ccs_ensemble_autotune": (ctypes.c_int, [ctypes.c_void_p, _I, _I, _D]),
     "ccs_ensemble_time_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
     "ccs_ensemble_synchronize": (ctypes.c_int, [ctypes.c_void_p]),

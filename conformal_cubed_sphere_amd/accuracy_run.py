@@ -2,22 +2,27 @@
 
 ``python -m conformal_cubed_sphere_amd.accuracy_run --n 720 --days 5``
 
-Runs the seeded balanced jet with 8 bumps and a Gaussian mountain (SSP-RK3,
-CFL 0.5, a = 6.371e6 m) three times from the same start:
-- FP64 (reference arithmetic);
+Runs one initial state (SSP-RK3, CFL 0.5, a = 6.371e6 m) several times:
+- FP64 (reference arithmetic: the parity kernel, or the split kernel in FP64);
 - FP32 split precision;
-- FP64 started from a state perturbed at FP32 rounding level, 2^-24 relative.
-  Its distance from the FP64 run is the method's own sensitivity floor.
+- FP64 started from the state perturbed by perturb_state at relative sizes
+  2^-k for each k of --perturb (default 24: one FP32 rounding).  The
+  perturbation is halo-consistent: every halo cell is perturbed exactly as
+  the interior cell it copies.  The distance of such a run from the FP64 run
+  is the method's own sensitivity to perturbations of that size.
 
-Afterwards it prints one JSON line with:
-- the normalised L1/L2/Linf of h, zonal u and meridional v (area-weighted, as
-  doc/orderCheck.m:55-74) for FP32 and for the perturbed run, both against FP64;
-- the relative mass and energy drift of each run;
-- the throughput of each run.
+The initial state is the seeded balanced jet with 8 bumps and a Gaussian
+mountain (--case 7, the benchmark recipe) or Williamson case 2, the steady
+zonal flow of test_case_mod.F90:61-95 (--case 2), whose exact solution does
+not change in time, so rounding errors are not amplified by developing
+structure.
 
-Progress lines go to stderr.  `--only fp32` runs the FP32 path alone and
-reports its drifts, throughput and the range of h (for runs too long to
-repeat in FP64).
+Every run records, at --snapshots evenly spaced times, the normalised L2 of h
+against the FP64 run at the same time; at the end the normalised L1/L2/Linf of
+h, zonal u and meridional v (area-weighted, as doc/orderCheck.m:55-74).  One
+JSON line is printed at the end; progress lines go to stderr.  `--only fp32`
+runs the FP32 path alone and reports its drifts, throughput and the range of h
+(for runs too long to repeat in FP64).
 """
 from __future__ import annotations
 
@@ -28,26 +33,39 @@ import time
 
 import numpy as np
 
-from .core import GRAVITY, DynamicalCore, Mesh, cfl_time_step, error_norms, init_case
+from .core import GRAVITY, DynamicalCore, Mesh, cfl_time_step, error_norms, init_case, perturb_state
 
 
-def _run(mesh, s0, dt, nsteps, precision, with_companions, chunk, label, fp64_ref=None):
+def _run(mesh, s0, dt, nsteps, precision, with_companions, chunk, label, marks=(), ref_h=None):
+    """Run nsteps; at each step count in marks record interior h (ref_h None)
+    or its normalised L2 against ref_h[mark]."""
     t0 = time.perf_counter()
+    area = mesh.interior(mesh.areaCell)
+    series = {}
     with DynamicalCore(mesh, dt, precision=precision) as core:
         core.set_state(s0, with_companions=with_companions)
         m0, e0 = core.diagnostics()
         done = 0
-        while done < nsteps:
-            k = min(chunk, nsteps - done)
-            core.step(k)
-            core.synchronize()
-            done += k
-            print(f"{label}: {done}/{nsteps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        stops = sorted(set(list(marks) + [nsteps]))
+        for stop in stops:
+            while done < stop:
+                k = min(chunk, stop - done)
+                core.step(k)
+                core.synchronize()
+                done += k
+                print(f"{label}: {done}/{nsteps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
+            if stop in marks:
+                h = mesh.interior(core.get_state().phi) / GRAVITY
+                if ref_h is None:
+                    series[stop] = h.copy()
+                else:
+                    series[stop] = error_norms(h, ref_h[stop], area)[1]
         m1, e1 = core.diagnostics()
         st = core.get_state()
     wall = time.perf_counter() - t0
-    return st, {"mass_drift": (m1 - m0) / m0, "energy_drift": (e1 - e0) / e0,
-                "cell_updates_per_s": 6 * mesh.n * mesh.n * nsteps / wall, "wall_s": wall}
+    return st, series, {"mass_drift": (m1 - m0) / m0, "energy_drift": (e1 - e0) / e0,
+                        "cell_updates_per_s": 6 * mesh.n * mesh.n * nsteps / wall, "wall_s": wall}
 
 
 def _norms(mesh, st, ref):
@@ -63,38 +81,45 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=720)
     ap.add_argument("--days", type=float, default=5.0)
+    ap.add_argument("--case", type=int, default=7, choices=[2, 7],
+                    help="7: seeded jet + bumps + mountain (benchmark recipe); 2: Williamson steady zonal flow")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--reference", default="fp64", choices=["fp64", "fp64_fast"],
                     help="FP64 reference: the parity kernel or the split kernel instantiated in FP64")
-    ap.add_argument("--no-floor", action="store_true", help="skip the perturbed FP64 run")
+    ap.add_argument("--perturb", default="24",
+                    help="comma-separated k: FP64 runs from the state perturbed by 2^-k (empty: none)")
+    ap.add_argument("--snapshots", type=int, default=0, help="L2(h) against FP64 at this many times")
     ap.add_argument("--chunk", type=int, default=4000)
     ap.add_argument("--only", choices=["fp32"], default=None,
                     help="run only the FP32 path (drift, stability and throughput; no error norms)")
     args = ap.parse_args()
 
     mesh = Mesh(args.n, halo=3, radius=6.371e6, omega=7.292e-5)
-    s0 = init_case(mesh, case_num=7, seed=args.seed, n_bumps=8, mountain=True, gravity=GRAVITY)
+    if args.case == 7:
+        s0 = init_case(mesh, case_num=7, seed=args.seed, n_bumps=8, mountain=True, gravity=GRAVITY)
+    else:
+        s0 = init_case(mesh, case_num=2, gravity=GRAVITY)
     dt, nsteps = cfl_time_step(mesh, s0, 0.5, args.days * 86400.0)
     if args.only == "fp32":
-        f32, f32_info = _run(mesh, s0, dt, nsteps, "fp32", True, args.chunk, "fp32")
+        f32, _, f32_info = _run(mesh, s0, dt, nsteps, "fp32", True, args.chunk, "fp32")
         h = mesh.interior(f32.phi) / GRAVITY
-        out = {"n": args.n, "days": args.days, "steps": nsteps, "dt_s": dt, "fp32": f32_info,
+        out = {"n": args.n, "case": args.case, "days": args.days, "steps": nsteps, "dt_s": dt, "fp32": f32_info,
                "finite": bool(np.all(np.isfinite(h))), "h_min_m": float(np.min(h)), "h_max_m": float(np.max(h))}
         print(json.dumps(out), flush=True)
         return
-    ref, ref_info = _run(mesh, s0, dt, nsteps, args.reference, True, args.chunk, args.reference)
-    f32, f32_info = _run(mesh, s0, dt, nsteps, "fp32", True, args.chunk, "fp32")
-    out = {"n": args.n, "days": args.days, "steps": nsteps, "dt_s": dt, "reference": args.reference,
-           "fp32_vs_fp64": _norms(mesh, f32, ref), "fp32": f32_info, args.reference: ref_info}
-    if not args.no_floor:
-        rng = np.random.default_rng(2024)
-        sp = s0.copy()
-        for k in ("phiG", "u", "v"):
-            a = getattr(sp, k)
-            a *= 1.0 + 2.0 ** -24 * rng.standard_normal(a.shape)
-        pert, pert_info = _run(mesh, sp, dt, nsteps, args.reference, False, args.chunk, "perturbed")
-        out["perturbed_fp64_vs_fp64"] = _norms(mesh, pert, ref)
-        out["perturbed"] = pert_info
+    marks = [round(nsteps * (k + 1) / args.snapshots) for k in range(args.snapshots)] if args.snapshots else []
+    ref, ref_h, ref_info = _run(mesh, s0, dt, nsteps, args.reference, True, args.chunk, args.reference, marks)
+    f32, f32_l2, f32_info = _run(mesh, s0, dt, nsteps, "fp32", True, args.chunk, "fp32", marks, ref_h)
+    out = {"n": args.n, "case": args.case, "days": args.days, "steps": nsteps, "dt_s": dt,
+           "reference": args.reference, "snapshot_hours": [m * dt / 3600.0 for m in marks],
+           "fp32_vs_fp64": _norms(mesh, f32, ref), "fp32_l2_h_series": [f32_l2[m] for m in marks],
+           "fp32": f32_info, args.reference: ref_info, "perturbed": {}}
+    for k in [int(x) for x in args.perturb.split(",") if x.strip()]:
+        sp = perturb_state(mesh, s0, 2.0 ** -k)
+        pert, pert_l2, pert_info = _run(mesh, sp, dt, nsteps, args.reference, True, args.chunk,
+                                        f"perturbed 2^-{k}", marks, ref_h)
+        out["perturbed"][f"2^-{k}"] = {"vs_fp64": _norms(mesh, pert, ref), "l2_h_series": [pert_l2[m] for m in marks],
+                                       **pert_info}
     print(json.dumps(out), flush=True)
 
 

@@ -142,6 +142,49 @@ def init_case(mesh: Mesh, case_num: int = 7, seed: int = 1, n_bumps: int = 8, mo
     return State(**a)
 
 
+def _splitmix64(x):
+    """SplitMix64 finaliser, elementwise on uint64 arrays (wraps mod 2^64)."""
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    return x ^ (x >> np.uint64(31))
+
+
+def perturb_state(mesh: Mesh, s: State, rel: float, seed: int = 2024) -> State:
+    """Copy of s whose phiG (and phi) is multiplied by 1 + rel z1 and whose
+    winds (u, v and their contravariant and spherical companions) by
+    1 + rel z2, with z1, z2 standard normal numbers drawn per
+    cell from a hash of the cell's (lon, lat) bits and the seed.
+
+    A halo cell holds a copy of the interior cell at the same position (its
+    lon, lat are bit-for-bit copies, ghost_mod.f90:18-82), so it gets the same
+    factors as its source: phiG stays a copy, and (u, v), scaled by one factor
+    per cell, stays the rotation of its source's winds.  The perturbed state
+    therefore keeps its halos consistent, as an FP32 rounding of the interior
+    followed by the halo fill would."""
+    with np.errstate(over="ignore"):
+        key = (mesh.lon.view(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ mesh.lat.view(np.uint64) ^ \
+            np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        out = s.copy()
+        z = []
+        for stream in (1, 2, 3, 4):
+            r = _splitmix64(_splitmix64(key ^ np.uint64((stream * 0x632BE59BD9B4E019) & 0xFFFFFFFFFFFFFFFF)))
+            z.append(((r >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53)  # uniform in (0, 1)
+    rad = np.sqrt(-2.0 * np.log(z[0]))
+    z1 = rad * np.cos(2.0 * np.pi * z[1])
+    rad2 = np.sqrt(-2.0 * np.log(z[2]))
+    z2 = rad2 * np.cos(2.0 * np.pi * z[3])
+    # the companions follow: phi = phiG / sqrtG scales with phiG, and the
+    # contravariant and spherical winds are linear in (u, v)
+    f1, f2 = 1.0 + rel * z1, 1.0 + rel * z2
+    out.phiG, out.phi = s.phiG * f1, s.phi * f1
+    for k in ("u", "v", "uc", "vc", "zonal", "merid"):
+        a = getattr(s, k)
+        if a is not None:
+            setattr(out, k, a * f2)
+    return out
+
+
 def cfl_time_step(mesh: Mesh, state: State, cfl: float, total_seconds: float) -> tuple[float, int]:
     """Largest dt <= cfl * min(dx / max lambda_x, dy / max lambda_y) that divides
     total_seconds, with the face eigenvalues of the Rusanov flux
@@ -342,13 +385,34 @@ class Ensemble:
         self._chk("ccs_ensemble_set_state", lib.ccs_ensemble_set_state(self._h, member, *(dptr(a) for a in args)))
 
     def autotune(self):
-        ty, ps, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        ty, ps, ms, cached = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
         self._chk("ccs_ensemble_autotune", lib.ccs_ensemble_autotune(self._h, ctypes.byref(ty), ctypes.byref(ps),
                                                                      ctypes.byref(ms)))
-        return {"tile_y": ty.value, "persistent": ps.value, "stage_ms": ms.value}
+        self._chk("ccs_ensemble_autotune_source", lib.ccs_ensemble_autotune_source(self._h, ctypes.byref(cached)))
+        return {"tile_y": ty.value, "persistent": ps.value, "stage_ms": ms.value, "from_cache": bool(cached.value)}
 
     def step(self, nsteps: int = 1):
+        """Advance every member nsteps steps of its own dt."""
         self._chk("ccs_ensemble_step", lib.ccs_ensemble_step(self._h, int(nsteps)))
+
+    def step_members(self, nsteps):
+        """Advance member k by nsteps[k] steps of its own dt, in the same
+        batched launches; members that have taken their steps retire."""
+        counts = [int(x) for x in nsteps]
+        if len(counts) != len(self):
+            raise ValueError("one step count per member")
+        arr = (ctypes.c_int * len(counts))(*counts)
+        self._chk("ccs_ensemble_step_members", lib.ccs_ensemble_step_members(self._h, arr))
+
+    def run_until(self, run_time: float):
+        """Advance every member to run_time seconds after its set_state:
+        member k reaches int(run_time / dt[k]) steps."""
+        self._chk("ccs_ensemble_run_until", lib.ccs_ensemble_run_until(self._h, float(run_time)))
+
+    def steps_done(self, member: int) -> int:
+        out = ctypes.c_longlong()
+        self._chk("ccs_ensemble_steps_done", lib.ccs_ensemble_steps_done(self._h, int(member), ctypes.byref(out)))
+        return out.value
 
     def synchronize(self):
         self._chk("ccs_ensemble_synchronize", lib.ccs_ensemble_synchronize(self._h))

@@ -13,9 +13,13 @@
 // stage after ccs_set_state reads the uploaded companions (initTestCase's).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,16 +50,20 @@ struct ccs_handle {
   double* face_speed = nullptr;  // CCS_OP_WENO_EXT: x-face then y-face G^-1 entries
   // split-precision path
   SplitState* split = nullptr;
-  // graphs of one step ([0]), of kGraphSteps steps ending with a rebase ([1])
-  // and of one step followed by a rebase ([2]); every step leaves the state
-  // in the same buffer, so a graph of several steps is the one-step sequence
-  // repeated
-  hipGraph_t graph[3]{};
-  hipGraphExec_t graph_exec[3]{};
-  // steps since the last ccs_set_state: the split path rebases its
-  // geopotential deviation after every step whose count is a multiple of
-  // kGraphSteps, whatever the grouping of steps into calls
-  long long steps_done = 0;
+  // Step graphs.  Every step leaves the state in the same buffer, so a graph
+  // of several steps is the one-step sequence repeated.  The split path
+  // folds its geopotential deviation into the FP64 reference field after
+  // every step whose count since ccs_set_state is a multiple of kRebaseSteps,
+  // whatever the grouping of steps into calls: graph[0] is one step, graph[1]
+  // one step and the rebase, graph[2 + phase] kGraphSteps steps starting at
+  // a step count of phase modulo kRebaseSteps (with the rebase after step
+  // kRebaseSteps - phase).  All are captured by the first ccs_step, so no
+  // later call pays for a capture.
+  static constexpr int kGraphs = 2 + kRebaseSteps;
+  hipGraph_t graph[kGraphs]{};
+  hipGraphExec_t graph_exec[kGraphs]{};
+  bool graphs_ready = false;
+  long long steps_done = 0;  // steps since the last ccs_set_state
 };
 
 namespace {
@@ -65,6 +73,27 @@ const char* kVersion = "ccs-mi355x 0.1.0 (gfx950)";
 // Steps per replay of the long graph: one launch per kGraphSteps steps keeps
 // the host submission rate far above what the GPU consumes.
 constexpr int kGraphSteps = 16;
+static_assert(kGraphSteps == kRebaseSteps, "a long graph holds exactly one rebase");
+
+// HIP events destroyed on every exit path.
+struct EventSet {
+  std::vector<hipEvent_t> ev;
+  explicit EventSet(std::size_t n) : ev(n, nullptr) {}
+  EventSet(const EventSet&) = delete;
+  EventSet& operator=(const EventSet&) = delete;
+  ~EventSet() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  hipError_t create() {
+    for (hipEvent_t& e : ev) {
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  }
+  hipEvent_t& operator[](std::size_t k) { return ev[k]; }
+};
 
 int fail(ccs_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -142,7 +171,7 @@ int enqueue_step(ccs_handle* hd, bool expl, hipEvent_t* ev) {
 int stages_per_step(const ccs_handle* hd) { return hd->cfg.scheme == CCS_RK3_TVD ? 3 : 4; }
 
 bool rebase_after(const ccs_handle* hd, long long step_index) {  // step_index: 1-based count after the step
-  return hd->cfg.precision != CCS_FP64_PARITY && step_index % kGraphSteps == 0;
+  return hd->cfg.precision != CCS_FP64_PARITY && step_index % kRebaseSteps == 0;
 }
 
 // one step, then the rebase if the step count reaches a multiple of kGraphSteps
@@ -154,18 +183,42 @@ int enqueue_counted_step(ccs_handle* hd, bool expl, hipEvent_t* ev) {
   return CCS_OK;
 }
 
+// Capture graph g of the handle (see ccs_handle::graph).
 int capture_graph(ccs_handle* hd, int g) {
   if (hd->graph_exec[g]) return CCS_OK;
+  const bool split = hd->cfg.precision != CCS_FP64_PARITY;
+  const int nsteps = g < 2 ? 1 : kGraphSteps;
+  // the step after which the rebase runs (1-based within the graph), 0: none
+  const int rebase_at = !split ? 0 : g == 0 ? 0 : g == 1 ? 1 : kRebaseSteps - (g - 2);
   HIPCHK(hd, hipStreamBeginCapture(hd->stream, hipStreamCaptureModeThreadLocal));
   int rc = CCS_OK;
-  for (int s = 0; s < (g == 1 ? kGraphSteps : 1) && rc == CCS_OK; ++s) rc = enqueue_step(hd, false, nullptr);
-  if (rc == CCS_OK && g != 0 && hd->cfg.precision != CCS_FP64_PARITY) split_enqueue_rebase(hd->split, hd->stream);
+  for (int s = 1; s <= nsteps && rc == CCS_OK; ++s) {
+    rc = enqueue_step(hd, false, nullptr);
+    if (rc == CCS_OK && s == rebase_at) split_enqueue_rebase(hd->split, hd->stream);
+  }
   hipGraph_t gr = nullptr;
   const hipError_t ec = hipStreamEndCapture(hd->stream, &gr);
-  if (rc != CCS_OK) return rc;
+  if (rc != CCS_OK) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
   if (ec != hipSuccess) return fail(hd, CCS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
   hd->graph[g] = gr;
   HIPCHK(hd, hipGraphInstantiate(&hd->graph_exec[g], hd->graph[g], nullptr, nullptr, 0));
+  return CCS_OK;
+}
+
+// Every graph a ccs_step may replay: 1-step, 1-step + rebase and the long
+// graph of each phase (split path), 1-step and one long graph (FP64 parity).
+int capture_graphs(ccs_handle* hd) {
+  if (hd->graphs_ready) return CCS_OK;
+  const bool split = hd->cfg.precision != CCS_FP64_PARITY;
+  for (int g = 0; g < ccs_handle::kGraphs; ++g) {
+    if (!split && (g == 1 || g > 2)) continue;
+    const int rc = capture_graph(hd, g);
+    if (rc != CCS_OK) return rc;
+  }
+  hd->graphs_ready = true;
   return CCS_OK;
 }
 
@@ -219,7 +272,9 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
   }
   int rc = CCS_OK;
   if (cfg->precision == CCS_FP64_PARITY) {
-    // statics 15 + state 9 + explicit 3 + tendencies 9 (interior) + derived 5
+    // statics 15 + state 9 + explicit 3 + derived 5 (padded), tendencies 9
+    // (interior), diagnostics partials and result; the carving below walks
+    // the same list and must end exactly at the end of the allocation
     const std::size_t np = hd->np, ni = hd->ni;
     const std::size_t total = (15 + 9 + 3 + 5) * np + 9 * ni + 4 * kDiagBlocks + 2;
     if (hipMalloc(&hd->dmem, total * sizeof(double)) != hipSuccess) {
@@ -244,6 +299,10 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
     for (int k = 0; k < 9; ++k) hd->tend[k] = take(ni);
     hd->partials = take(4 * kDiagBlocks);
     hd->result = take(2);
+    if (static_cast<std::size_t>(p - hd->dmem) != total) {  // the carving and the size disagree
+      ccs_destroy(hd);
+      return CCS_ERR_INVALID;
+    }
   } else {
     rc = split_create(*cfg, hd->hs, &hd->split, &hd->err);
     if (rc != CCS_OK) {
@@ -258,7 +317,7 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
 extern "C" void ccs_destroy(ccs_handle* hd) {
   if (!hd) return;
   // teardown: errors here have nowhere to go
-  for (int g = 0; g < 3; ++g) {
+  for (int g = 0; g < ccs_handle::kGraphs; ++g) {
     if (hd->graph_exec[g]) (void)hipGraphExecDestroy(hd->graph_exec[g]);
     if (hd->graph[g]) (void)hipGraphDestroy(hd->graph[g]);
   }
@@ -400,17 +459,20 @@ extern "C" int ccs_step(ccs_handle* hd, int nsteps) {
     hd->explicit_valid = false;
     k = 1;
   }
-  if (hd->cfg.use_graph) {
-    // 16-step graphs on steps aligned to multiples of kGraphSteps (each ends
-    // with the rebase), single steps elsewhere
+  if (hd->cfg.use_graph && k < nsteps) {
+    const int rc = capture_graphs(hd);
+    if (rc != CCS_OK) return rc;
+    const bool split = hd->cfg.precision != CCS_FP64_PARITY;
     while (k < nsteps) {
-      int g;
-      if (hd->steps_done % kGraphSteps == 0 && nsteps - k >= kGraphSteps) g = 1;
-      else g = rebase_after(hd, hd->steps_done + 1) ? 2 : 0;
-      const int rc = capture_graph(hd, g);
-      if (rc != CCS_OK) return rc;
+      int g, adv;
+      if (nsteps - k >= kGraphSteps) {
+        g = 2 + (split ? static_cast<int>(hd->steps_done % kRebaseSteps) : 0);
+        adv = kGraphSteps;
+      } else {
+        g = rebase_after(hd, hd->steps_done + 1) ? 1 : 0;
+        adv = 1;
+      }
       HIPCHK(hd, hipGraphLaunch(hd->graph_exec[g], hd->stream));
-      const int adv = (g == 1) ? kGraphSteps : 1;
       hd->steps_done += adv;
       k += adv;
     }
@@ -518,24 +580,23 @@ extern "C" int ccs_time_steps(ccs_handle* hd, int nsteps, double* total_ms, doub
     const int rc = ccs_step(hd, 1);
     if (rc != CCS_OK) return rc;
   }
-  hipEvent_t e0, e1;
-  HIPCHK(hd, hipEventCreate(&e0));
-  HIPCHK(hd, hipEventCreate(&e1));
-  HIPCHK(hd, hipEventRecord(e0, hd->stream));
+  EventSet ends(2);
+  HIPCHK(hd, ends.create());
+  HIPCHK(hd, hipEventRecord(ends[0], hd->stream));
   int rc = ccs_step(hd, nsteps);
-  HIPCHK(hd, hipEventRecord(e1, hd->stream));
-  HIPCHK(hd, hipEventSynchronize(e1));
+  HIPCHK(hd, hipEventRecord(ends[1], hd->stream));
+  HIPCHK(hd, hipEventSynchronize(ends[1]));
   float ms = 0.f;
-  HIPCHK(hd, hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hd, hipEventElapsedTime(&ms, ends[0], ends[1]));
   if (total_ms) *total_ms = ms;
   if (rc == CCS_OK && kernel_ms) {
     // Second pass: events around every stage launch (no graph), mean duration.
     const int ns = stages_per_step(hd);
-    std::vector<hipEvent_t> ev(2 * ns);
-    for (auto& e : ev) HIPCHK(hd, hipEventCreate(&e));
+    EventSet ev(2 * ns);
+    HIPCHK(hd, ev.create());
     double acc = 0.0;
     for (int k = 0; k < nsteps && rc == CCS_OK; ++k) {
-      rc = enqueue_counted_step(hd, false, ev.data());
+      rc = enqueue_counted_step(hd, false, ev.ev.data());
       HIPCHK(hd, hipEventSynchronize(ev[2 * ns - 1]));
       for (int s = 0; s < ns; ++s) {
         float t = 0.f;
@@ -543,11 +604,8 @@ extern "C" int ccs_time_steps(ccs_handle* hd, int nsteps, double* total_ms, doub
         acc += t;
       }
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     *kernel_ms = acc / (static_cast<double>(nsteps) * ns);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return rc;
 }
 
@@ -560,11 +618,13 @@ struct ccs_ensemble {
   hipStream_t stream = nullptr;
   std::string err;
   EnsPlan* plan = nullptr;
-  int tile_y = 0, persistent = 0;
+  int tile_y = 0, persistent = 0, tuned_from_cache = 0;
   double stage_ms = 0.0;
-  hipGraph_t graph[3]{};       // as ccs_handle: one step, 16 steps + rebase, one step + rebase
-  hipGraphExec_t graph_exec[3]{};
-  long long steps_done = 0;    // steps since the last ccs_ensemble_set_state
+  // one step ([0]) and kGraphSteps steps ([1]) of a run: every launch of a
+  // step is the same for all steps (the run's step index lives on the device)
+  hipGraph_t graph[2]{};
+  hipGraphExec_t graph_exec[2]{};
+  std::vector<long long> done;  // per member: steps since its last ccs_ensemble_set_state
 };
 
 namespace {
@@ -582,7 +642,7 @@ int efail(ccs_ensemble* e, int code, const std::string& msg) {
   } while (0)
 
 void ens_drop_graphs(ccs_ensemble* e) {
-  for (int g = 0; g < 3; ++g) {
+  for (int g = 0; g < 2; ++g) {
     if (e->graph_exec[g]) (void)hipGraphExecDestroy(e->graph_exec[g]);
     if (e->graph[g]) (void)hipGraphDestroy(e->graph[g]);
     e->graph_exec[g] = nullptr;
@@ -609,6 +669,115 @@ std::vector<SplitState*> ens_states(const ccs_ensemble* e) {
   return s;
 }
 
+// ---- autotune cache: the choice per (device, precision, scheme, halo,
+// member sizes, constraints of the config), kept for the life of the process
+// and, when CCS_AUTOTUNE_CACHE names a file, in that file (one line per
+// choice: key tile_y persistent stage_ms).
+struct TuneChoice {
+  int tile_y = 16, persistent = 0;
+  double stage_ms = 0.0;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune;
+bool g_tune_loaded = false;
+
+std::string tune_key(const ccs_ensemble* e) {
+  hipDeviceProp_t prop{};
+  std::string arch = "dev";
+  if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess)
+    arch = std::string(prop.gcnArchName) + "x" + std::to_string(prop.multiProcessorCount);
+  for (char& c : arch)
+    if (c == ' ' || c == ':') c = '_';
+  std::map<int, int> sizes;
+  for (const ccs_handle* h : e->m) ++sizes[h->n];
+  std::string k = arch + ":p" + std::to_string(e->cfg.precision) + ":s" + std::to_string(e->cfg.scheme) + ":h" +
+                  std::to_string(e->cfg.halo) + ":t" + std::to_string(e->cfg.tile_y) + ":g" +
+                  std::to_string(e->cfg.persistent) + ":";
+  for (const auto& kv : sizes) k += std::to_string(kv.first) + "x" + std::to_string(kv.second) + ",";
+  return k;
+}
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* path = std::getenv("CCS_AUTOTUNE_CACHE");
+  if (!path || !*path) return;
+  std::FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  char key[4096];
+  TuneChoice c;
+  while (std::fscanf(f, "%4095s %d %d %lf", key, &c.tile_y, &c.persistent, &c.stage_ms) == 4)
+    if ((c.tile_y == 8 || c.tile_y == 16 || c.tile_y == 32) && (c.persistent == 0 || c.persistent == 1))
+      g_tune[key] = c;
+  std::fclose(f);
+}
+
+bool tune_lookup(const std::string& key, TuneChoice* c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  const auto it = g_tune.find(key);
+  if (it == g_tune.end()) return false;
+  *c = it->second;
+  return true;
+}
+
+void tune_store(const std::string& key, const TuneChoice& c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  g_tune[key] = c;
+  const char* path = std::getenv("CCS_AUTOTUNE_CACHE");
+  if (!path || !*path) return;
+  if (std::FILE* f = std::fopen(path, "a")) {
+    std::fprintf(f, "%s %d %d %.9g\n", key.c_str(), c.tile_y, c.persistent, c.stage_ms);
+    std::fclose(f);
+  }
+}
+
+// Advance member m by target[m] steps: one batched launch per RK stage for
+// all members still stepping, replayed from the one-step and kGraphSteps-step
+// graphs.
+int ens_run(ccs_ensemble* e, const int* target) {
+  for (ccs_handle* h : e->m)
+    if (!h->mesh_set || !h->state_set)
+      return efail(e, CCS_ERR_ORDER, "ensemble run before every member's mesh and state are set");
+  int steps = 0;
+  for (std::size_t k = 0; k < e->m.size(); ++k) {
+    if (target[k] < 0) return efail(e, CCS_ERR_INVALID, "ensemble: negative step count");
+    steps = std::max(steps, target[k]);
+  }
+  if (!e->plan) {
+    const int rc = ccs_ensemble_autotune(e, nullptr, nullptr, nullptr);
+    if (rc != CCS_OK) return rc;
+  }
+  ENSCHK(e, hipSetDevice(e->cfg.device));
+  if (steps == 0) return CCS_OK;
+  // members' own streams (uploads) are idle: every member call synchronises
+  int rc = ens_plan_set_run(e->plan, e->stream, target, &e->err);
+  if (rc != CCS_OK) return rc;
+  const int scheme = e->cfg.scheme;
+  int k = 0;
+  if (e->cfg.use_graph) {
+    for (int g = 0; g < 2; ++g) {
+      if (e->graph_exec[g]) continue;
+      ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < (g == 1 ? kGraphSteps : 1); ++s) ens_enqueue_step(e->plan, e->stream, scheme);
+      hipGraph_t gr = nullptr;
+      ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
+      e->graph[g] = gr;
+      ENSCHK(e, hipGraphInstantiate(&e->graph_exec[g], e->graph[g], nullptr, nullptr, 0));
+    }
+    while (k < steps) {
+      const int g = (steps - k >= kGraphSteps) ? 1 : 0;
+      ENSCHK(e, hipGraphLaunch(e->graph_exec[g], e->stream));
+      k += g ? kGraphSteps : 1;
+    }
+  }
+  for (; k < steps; ++k) ens_enqueue_step(e->plan, e->stream, scheme);
+  ENSCHK(e, hipGetLastError());
+  for (std::size_t m = 0; m < e->m.size(); ++m) e->done[m] += target[m];
+  return CCS_OK;
+}
+
 }  // namespace
 
 extern "C" const char* ccs_ensemble_last_error(const ccs_ensemble* e) {
@@ -629,6 +798,7 @@ extern "C" int ccs_ensemble_create(const ccs_ensemble_config* cfg, const int* n,
     if (n[k] < 4 || n[k] < cfg->halo || !(dt[k] > 0.0) || !(dx[k] > 0.0)) return CCS_ERR_INVALID;
   ccs_ensemble* e = new ccs_ensemble();
   e->cfg = *cfg;
+  e->done.assign(cfg->n_members, 0);
   for (int k = 0; k < cfg->n_members; ++k) {
     ccs_config c{};
     c.n = n[k];
@@ -684,8 +854,13 @@ extern "C" int ccs_ensemble_set_state(ccs_ensemble* e, int member, const double*
   int rc = ens_member(e, member);
   if (rc != CCS_OK) return rc;
   ENSCHK(e, hipStreamSynchronize(e->stream));
-  e->steps_done = 0;  // rebase points count from the last upload
-  return ens_forward(e, member, ccs_set_state(e->m[member], phiG, u, v, nullptr, nullptr, nullptr));
+  rc = ens_forward(e, member, ccs_set_state(e->m[member], phiG, u, v, nullptr, nullptr, nullptr));
+  if (rc != CCS_OK) return rc;
+  // this member's rebase points count from this upload; the others keep theirs
+  e->done[member] = 0;
+  if (e->plan && ens_plan_reset_member(e->plan, e->stream, member) != CCS_OK)
+    return efail(e, CCS_ERR_DEVICE, "ccs_ensemble_set_state: step counter reset failed");
+  return CCS_OK;
 }
 
 extern "C" int ccs_ensemble_get_state(ccs_ensemble* e, int member, double* phiG, double* u, double* v, double* phi,
@@ -703,6 +878,12 @@ extern "C" int ccs_ensemble_diagnostics(ccs_ensemble* e, int member, double* mas
   return ens_forward(e, member, ccs_diagnostics(e->m[member], mass, energy));
 }
 
+extern "C" int ccs_ensemble_steps_done(const ccs_ensemble* e, int member, long long* steps) {
+  if (!e || !steps || member < 0 || member >= static_cast<int>(e->m.size())) return CCS_ERR_INVALID;
+  *steps = e->done[member];
+  return CCS_OK;
+}
+
 extern "C" int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persistent, double* stage_ms) {
   if (!e) return CCS_ERR_INVALID;
   for (ccs_handle* h : e->m)
@@ -710,48 +891,57 @@ extern "C" int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persiste
   ENSCHK(e, hipSetDevice(e->cfg.device));
   ENSCHK(e, hipStreamSynchronize(e->stream));
   std::vector<SplitState*> st = ens_states(e);
-  std::vector<int> tys, pers;
-  if (e->cfg.tile_y) tys.push_back(e->cfg.tile_y); else tys = {8, 16, 32};
-  if (e->cfg.persistent >= 0) pers.push_back(e->cfg.persistent); else pers = {0, 1};
-  hipEvent_t e0, e1;
-  ENSCHK(e, hipEventCreate(&e0));
-  ENSCHK(e, hipEventCreate(&e1));
+  const int nm = static_cast<int>(st.size());
+  const std::string key = tune_key(e);
+  TuneChoice cached;
   EnsPlan* best = nullptr;
   double best_ms = 0.0;
   int rc = CCS_OK;
-  constexpr int kReps = 6;
-  for (int ty : tys) {
-    for (int ps : pers) {
-      EnsPlan* p = nullptr;
-      rc = ens_plan_create(st.data(), static_cast<int>(st.size()), ty, ps, e->cfg.device, &p, &e->err);
+  if (tune_lookup(key, &cached)) {
+    rc = ens_plan_create(st.data(), nm, cached.tile_y, cached.persistent, e->cfg.device, e->done.data(), &best, &e->err);
+    if (rc != CCS_OK) return rc;
+    best_ms = cached.stage_ms;
+    e->tuned_from_cache = 1;
+  } else {
+    std::vector<int> tys, pers;
+    if (e->cfg.tile_y) tys.push_back(e->cfg.tile_y); else tys = {8, 16, 32};
+    if (e->cfg.persistent >= 0) pers.push_back(e->cfg.persistent); else pers = {0, 1};
+    EventSet ev(2);
+    ENSCHK(e, ev.create());
+    constexpr int kReps = 6;
+    for (int ty : tys) {
+      for (int ps : pers) {
+        EnsPlan* p = nullptr;
+        rc = ens_plan_create(st.data(), nm, ty, ps, e->cfg.device, e->done.data(), &p, &e->err);
+        if (rc != CCS_OK) break;
+        // the tendency launch has the cost of a stage and changes no state
+        ens_enqueue_tendency(p, e->stream);
+        (void)hipEventRecord(ev[0], e->stream);
+        for (int r = 0; r < kReps; ++r) ens_enqueue_tendency(p, e->stream);
+        (void)hipEventRecord(ev[1], e->stream);
+        float ms = 0.f;
+        if (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+          ens_plan_destroy(p);
+          rc = efail(e, CCS_ERR_DEVICE, "ccs_ensemble_autotune: timing failed");
+          break;
+        }
+        const double mean = ms / kReps;
+        if (!best || mean < best_ms) {
+          if (best) ens_plan_destroy(best);
+          best = p;
+          best_ms = mean;
+        } else {
+          ens_plan_destroy(p);
+        }
+      }
       if (rc != CCS_OK) break;
-      // the tendency launch has the cost of a stage and changes no state
-      ens_enqueue_tendency(p, e->stream);
-      (void)hipEventRecord(e0, e->stream);
-      for (int r = 0; r < kReps; ++r) ens_enqueue_tendency(p, e->stream);
-      (void)hipEventRecord(e1, e->stream);
-      float ms = 0.f;
-      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        ens_plan_destroy(p);
-        rc = efail(e, CCS_ERR_DEVICE, "ccs_ensemble_autotune: timing failed");
-        break;
-      }
-      const double mean = ms / kReps;
-      if (!best || mean < best_ms) {
-        if (best) ens_plan_destroy(best);
-        best = p;
-        best_ms = mean;
-      } else {
-        ens_plan_destroy(p);
-      }
     }
-    if (rc != CCS_OK) break;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc != CCS_OK) {
-    if (best) ens_plan_destroy(best);
-    return rc;
+    if (rc != CCS_OK) {
+      if (best) ens_plan_destroy(best);
+      return rc;
+    }
+    tune_store(key, TuneChoice{ens_plan_tile_y(best), ens_plan_persistent(best), best_ms});
+    e->tuned_from_cache = 0;
   }
   ens_drop_graphs(e);
   if (e->plan) ens_plan_destroy(e->plan);
@@ -765,47 +955,37 @@ extern "C" int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persiste
   return CCS_OK;
 }
 
+extern "C" int ccs_ensemble_autotune_source(const ccs_ensemble* e, int* from_cache) {
+  if (!e || !from_cache) return CCS_ERR_INVALID;
+  *from_cache = e->tuned_from_cache;
+  return CCS_OK;
+}
+
 extern "C" int ccs_ensemble_step(ccs_ensemble* e, int nsteps) {
   if (!e) return CCS_ERR_INVALID;
   if (nsteps < 0) return efail(e, CCS_ERR_INVALID, "ccs_ensemble_step: nsteps < 0");
-  for (ccs_handle* h : e->m)
-    if (!h->mesh_set || !h->state_set)
-      return efail(e, CCS_ERR_ORDER, "ccs_ensemble_step before every member's mesh and state are set");
-  if (!e->plan) {
-    const int rc = ccs_ensemble_autotune(e, nullptr, nullptr, nullptr);
-    if (rc != CCS_OK) return rc;
+  const std::vector<int> target(e->m.size(), nsteps);
+  return ens_run(e, target.data());
+}
+
+extern "C" int ccs_ensemble_step_members(ccs_ensemble* e, const int* nsteps) {
+  if (!e) return CCS_ERR_INVALID;
+  if (!nsteps) return efail(e, CCS_ERR_INVALID, "ccs_ensemble_step_members: null step counts");
+  return ens_run(e, nsteps);
+}
+
+extern "C" int ccs_ensemble_run_until(ccs_ensemble* e, double run_time) {
+  if (!e) return CCS_ERR_INVALID;
+  if (!(run_time >= 0.0)) return efail(e, CCS_ERR_INVALID, "ccs_ensemble_run_until: run_time must be >= 0");
+  std::vector<int> target(e->m.size());
+  for (std::size_t k = 0; k < e->m.size(); ++k) {
+    // integer conversion of run_time / dt, as parameters_mod.F90:149 derives nsteps
+    const double q = run_time / e->m[k]->cfg.dt;
+    if (!(q < 2147483647.0)) return efail(e, CCS_ERR_INVALID, "ccs_ensemble_run_until: too many steps");
+    const long long total = static_cast<long long>(q);
+    target[k] = static_cast<int>(std::max(0LL, total - e->done[k]));
   }
-  ENSCHK(e, hipSetDevice(e->cfg.device));
-  // members' own streams (uploads) are idle: every member call synchronises
-  const int scheme = e->cfg.scheme;
-  auto rebase_due = [](long long step_index) { return step_index % kGraphSteps == 0; };
-  int k = 0;
-  if (e->cfg.use_graph) {
-    while (k < nsteps) {
-      int g;
-      if (e->steps_done % kGraphSteps == 0 && nsteps - k >= kGraphSteps) g = 1;
-      else g = rebase_due(e->steps_done + 1) ? 2 : 0;
-      if (!e->graph_exec[g]) {
-        ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < (g == 1 ? kGraphSteps : 1); ++s) ens_enqueue_step(e->plan, e->stream, scheme);
-        if (g != 0) ens_enqueue_rebase(e->plan, e->stream);
-        hipGraph_t gr = nullptr;
-        ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
-        e->graph[g] = gr;
-        ENSCHK(e, hipGraphInstantiate(&e->graph_exec[g], e->graph[g], nullptr, nullptr, 0));
-      }
-      ENSCHK(e, hipGraphLaunch(e->graph_exec[g], e->stream));
-      const int adv = (g == 1) ? kGraphSteps : 1;
-      e->steps_done += adv;
-      k += adv;
-    }
-  }
-  for (; k < nsteps; ++k) {
-    ens_enqueue_step(e->plan, e->stream, scheme);
-    if (rebase_due(++e->steps_done)) ens_enqueue_rebase(e->plan, e->stream);
-  }
-  ENSCHK(e, hipGetLastError());
-  return CCS_OK;
+  return ens_run(e, target.data());
 }
 
 extern "C" int ccs_ensemble_time_steps(ccs_ensemble* e, int nsteps, double* total_ms) {
@@ -815,18 +995,20 @@ extern "C" int ccs_ensemble_time_steps(ccs_ensemble* e, int nsteps, double* tota
     const int rc = ccs_ensemble_autotune(e, nullptr, nullptr, nullptr);
     if (rc != CCS_OK) return rc;
   }
-  hipEvent_t e0, e1;
-  ENSCHK(e, hipEventCreate(&e0));
-  ENSCHK(e, hipEventCreate(&e1));
-  ENSCHK(e, hipEventRecord(e0, e->stream));
-  const int rc = ccs_ensemble_step(e, nsteps);
-  ENSCHK(e, hipEventRecord(e1, e->stream));
-  ENSCHK(e, hipEventSynchronize(e1));
+  // a run of nsteps for every member is set up (and its tables uploaded)
+  // before the timed region
+  const std::vector<int> target(e->m.size(), nsteps);
+  int rc = ens_plan_set_run(e->plan, e->stream, target.data(), &e->err);
+  if (rc != CCS_OK) return rc;
+  EventSet ev(2);
+  ENSCHK(e, ev.create());
+  ENSCHK(e, hipEventRecord(ev[0], e->stream));
+  rc = ccs_ensemble_step(e, nsteps);
+  ENSCHK(e, hipEventRecord(ev[1], e->stream));
+  ENSCHK(e, hipEventSynchronize(ev[1]));
   float ms = 0.f;
-  ENSCHK(e, hipEventElapsedTime(&ms, e0, e1));
+  ENSCHK(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
   if (total_ms) *total_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return rc;
 }
 

@@ -54,6 +54,7 @@
 #include <vector>
 
 #include "ccs_internal.h"
+#include "ccs_share.h"
 #include "ccs_split.h"
 
 namespace ccs {
@@ -584,6 +585,8 @@ struct EnsMember {
   T* tend[9];
   T inc_half, inc_full;  // RK4 increments dt/(2a), dt/a
   int tile0;             // first global tile of the member
+  int target;            // steps the member takes in the current run
+  int member;            // index of the member in the ensemble (its slot in the step counters)
   int hs[6][4][11];
   double* pref64;        // FP64 reference field and its cast (rebase)
   T* pref_rw;
@@ -596,11 +599,21 @@ struct EnsStage {
   int inc_full; // RK4: 1 -> dt/a increment, 0 -> dt/(2a)
 };
 
+// Step index within the current run (device counter, advanced by ens_tick),
+// or -1 where every member steps (the tendency launch of the autotuner).
+__device__ __forceinline__ int run_step(const int* run_c) {
+  return run_c ? __hip_atomic_load(run_c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
+}
+
+// One tile of the global list; false when its member has taken its steps
+// (members are ordered by decreasing step count, so every later tile of the
+// list is finished too).
 template <typename T, int KIND, int TY>
-__device__ __forceinline__ void ens_tile(const EnsMember<T>* __restrict__ md, const int* __restrict__ tile_member,
-                                         const EnsStage& sel, const unsigned t, const int tid) {
+__device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, const int* __restrict__ tile_member,
+                                         const EnsStage& sel, const unsigned t, const int tid, const int step) {
   const int m = tile_member[t];
   const EnsMember<T>* M = md + m;
+  if (step >= M->target) return false;
   SplitArgs<T> a = M->a;
   a.in = Prog<const T>{M->buf[sel.in_b][0], M->buf[sel.in_b][1], M->buf[sel.in_b][2]};
   a.base = Prog<const T>{M->buf[sel.base_b][0], M->buf[sel.base_b][1], M->buf[sel.base_b][2]};
@@ -614,30 +627,30 @@ __device__ __forceinline__ void ens_tile(const EnsMember<T>* __restrict__ md, co
   const int i0 = static_cast<int>(lt - tq * a.nbx) * TX;
   const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
   stage_tile<T, KIND, TY, 0>(a, M->st, M->hs[p], p, i0, j0, tid);
+  return true;
 }
 
 // PERSIST = false: one workgroup per tile, in the XCD-aware order.
-// PERSIST = true: a grid of resident workgroups; each XCD's workgroups take
-// turns through that XCD's contiguous share of the global tile list.
+// PERSIST = true: a grid of resident workgroups looping over their share of
+// the global tile list (ens_share).
 template <typename T, int KIND, int TY, bool PERSIST>
 __global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restrict__ md,
-                                                      const int* __restrict__ tile_member, int ntiles, EnsStage sel) {
+                                                      const int* __restrict__ tile_member, int ntiles, EnsStage sel,
+                                                      const int* run_c) {
   const unsigned nb = gridDim.x, b = blockIdx.x;
+  const int step = run_step(run_c);
   if (!PERSIST) {
-    ens_tile<T, KIND, TY>(md, tile_member, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x));
+    (void)ens_tile<T, KIND, TY>(md, tile_member, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x), step);
     return;
   }
-  const unsigned xcd = b & 7u;
-  const unsigned per = (nb >> 3) + (xcd < (nb & 7u) ? 1u : 0u);  // workgroups on this XCD
-  const unsigned lo = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * xcd) >> 3);
-  const unsigned hi = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * (xcd + 1)) >> 3);
-  for (unsigned t = lo + (b >> 3); t < hi; t += per) {
+  const EnsShare sh = ens_share(b, nb, static_cast<unsigned>(ntiles));
+  for (unsigned t = sh.first; t < sh.end; t += sh.stride) {
     __syncthreads();  // the previous tile's LDS reads are done
     // keep the thread-index arithmetic inside the loop: hoisting it out would
     // hold ~30 more VGPRs for the whole kernel and halve the occupancy
     int tid = static_cast<int>(threadIdx.x);
     asm volatile("" : "+v"(tid));
-    ens_tile<T, KIND, TY>(md, tile_member, sel, t, tid);
+    if (!ens_tile<T, KIND, TY>(md, tile_member, sel, t, tid, step)) break;
   }
 }
 
@@ -657,10 +670,16 @@ __global__ void __launch_bounds__(256) split_rebase(double* __restrict__ pref64,
   }
 }
 
-// the same for every member of an ensemble (blockIdx.y = member)
+// The same for the members of an ensemble (blockIdx.y = position in the
+// member table) whose own step count reaches a multiple of kRebaseSteps with
+// the step just taken: done[] holds each member's steps since its upload,
+// modulo kRebaseSteps (ens_tick advances it after this kernel).
 template <typename T>
-__global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict__ md) {
+__global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict__ md, const int* run_c,
+                                                  const int* done) {
   const EnsMember<T>* M = md + blockIdx.y;
+  const int c = run_step(run_c);
+  if (c >= M->target || (done[M->member] + 1) % kRebaseSteps != 0) return;
   T* dP = M->buf[0][0];
   for (unsigned long long d = blockIdx.x * 256ull + threadIdx.x; d < M->np; d += gridDim.x * 256ull) {
     const double p = M->pref64[d] + static_cast<double>(dP[d]);
@@ -668,6 +687,23 @@ __global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict
     M->pref_rw[d] = static_cast<T>(p);
     dP[d] = T(0);
   }
+}
+
+// End of a step: the step counters of the members that took it, then the
+// run's step index (one workgroup; every thread reads the index before
+// thread 0 advances it).
+template <typename T>
+__global__ void __launch_bounds__(256) ens_tick(const EnsMember<T>* __restrict__ md, int nm, int* run_c, int* done) {
+  const int c = run_step(run_c);
+  for (int k = threadIdx.x; k < nm; k += 256)
+    if (c < md[k].target) done[md[k].member] = (done[md[k].member] + 1) % kRebaseSteps;
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(run_c, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Start of a run: step index 0.
+__global__ void ens_run_start(int* run_c) {
+  if (threadIdx.x == 0) __hip_atomic_store(run_c, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Compensated FP64 mass/energy of the split state.
@@ -1006,24 +1042,45 @@ int split_set_mesh(SplitState* s, hipStream_t st, double* const T[15], std::stri
 
 int split_set_state(SplitState* s, hipStream_t st, const double* phiG, const double* u, const double* v,
                     const double*, const double*, const double*, std::string* err) {
+  const int n = s->n, h = s->h;
   const std::size_t np = s->np;
   const double r = s->cfg.radius;
   s->pref64.assign(np, 0.0);
-  std::vector<double> U(np), V(np), zero(np, 0.0);
+  std::vector<double> U(np), V(np), dP(np, 0.0);
   for (std::size_t d = 0; d < np; ++d) {
     s->pref64[d] = phiG[d] / (r * r);
     U[d] = u[d] / r;
     V[d] = v[d] / r;
   }
+  // Halo cells hold copies of interior cells (ghost_mod.f90:18-82).  After
+  // the first stage the kernel writes a halo cell's deviation dP from its
+  // source cell, so the halo's reference value must be the source's: halo
+  // Pref = source Pref, as fill_ghost makes it, and the uploaded halo value
+  // is kept for the first stage (which reads the state as uploaded) in the
+  // halo's deviation.
+  for (int p = 1; p <= 6; ++p)
+    for (int side = 0; side < 4; ++side) {
+      const HaloStrip& hsr = halo_strip(p, side);
+      for (int k = 1; k <= h; ++k)
+        for (int t = 1; t <= n; ++t) {
+          int di, dj, si, sj;
+          halo_dst(n, side, k, t, &di, &dj);
+          halo_src(n, hsr, k, t, &si, &sj);
+          const std::size_t dd = pidx(n, h, di, dj, p), ds = pidx(n, h, si, sj, hsr.q);
+          const double src = phiG[ds] / (r * r);
+          dP[dd] = s->pref64[dd] - src;
+          s->pref64[dd] = src;
+        }
+    }
   std::vector<char> stage;
   if (s->fp64) {
     upload_cast<double>(st, s->pref, s->pref64, stage);
-    upload_cast<double>(st, s->buf[0][0], zero, stage);
+    upload_cast<double>(st, s->buf[0][0], dP, stage);
     upload_cast<double>(st, s->buf[0][1], U, stage);
     upload_cast<double>(st, s->buf[0][2], V, stage);
   } else {
     upload_cast<float>(st, s->pref, s->pref64, stage);
-    upload_cast<float>(st, s->buf[0][0], zero, stage);
+    upload_cast<float>(st, s->buf[0][0], dP, stage);
     upload_cast<float>(st, s->buf[0][1], U, stage);
     upload_cast<float>(st, s->buf[0][2], V, stage);
   }
@@ -1150,39 +1207,44 @@ int split_diagnostics(SplitState* s, hipStream_t st, bool, double* result, std::
 struct EnsPlan {
   bool fp64 = false;
   int tile_y = 16, persistent = 0, grid = 0, ntiles = 0, nm = 0;
-  void* dmem = nullptr;     // member table, then the tile -> member map
+  void* dmem = nullptr;     // member table, tile -> member map, run step index, step counters
   void* d_desc = nullptr;
   int* d_tiles = nullptr;
+  int* d_run = nullptr;     // step index within the current run
+  int* d_done = nullptr;    // per member: steps since its upload, modulo kRebaseSteps
+  std::vector<char> md_nat;           // member table in member order (EnsMember<T>)
+  std::vector<int> tiles_per_member;
+  std::vector<int> target;            // targets of the table on the device (empty: none uploaded)
 };
 
 namespace {
 
 template <typename T, int TY, bool PERSIST>
-void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel) {
+void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run) {
   const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
   const dim3 grid(static_cast<unsigned>(e->grid)), block(TX * TY);
   const int nt = e->ntiles;
   switch (kind) {
-    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
-    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
-    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
-    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
-    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
-    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel); break;
+    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
+    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
+    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
   }
 }
 
 template <typename T, int TY>
-void ens_launch_ty(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel) {
-  if (e->persistent) ens_launch_k<T, TY, true>(e, s, kind, sel);
-  else ens_launch_k<T, TY, false>(e, s, kind, sel);
+void ens_launch_ty(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run) {
+  if (e->persistent) ens_launch_k<T, TY, true>(e, s, kind, sel, run);
+  else ens_launch_k<T, TY, false>(e, s, kind, sel, run);
 }
 
 template <typename T>
-void ens_launch(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel) {
-  if (e->tile_y == 8) ens_launch_ty<T, 8>(e, s, kind, sel);
-  else if (e->tile_y == 32) ens_launch_ty<T, 32>(e, s, kind, sel);
-  else ens_launch_ty<T, 16>(e, s, kind, sel);
+void ens_launch(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run) {
+  if (e->tile_y == 8) ens_launch_ty<T, 8>(e, s, kind, sel, run);
+  else if (e->tile_y == 32) ens_launch_ty<T, 32>(e, s, kind, sel, run);
+  else ens_launch_ty<T, 16>(e, s, kind, sel, run);
 }
 
 template <typename T, int TY>
@@ -1194,10 +1256,11 @@ int ens_occupancy() {
 }
 
 template <typename T>
-int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, std::string* err) {
+int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long long* done0, std::string* err) {
   const int TY = e->tile_y;
   std::vector<EnsMember<T>> md(nm);
   std::vector<int> tiles;
+  e->tiles_per_member.resize(nm);
   for (int m = 0; m < nm; ++m) {
     const SplitState* s = mem[m];
     EnsMember<T>& d = md[m];
@@ -1214,21 +1277,33 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, std::strin
     d.inc_half = static_cast<T>(0.5 * s->cfg.dt / s->cfg.radius);
     d.inc_full = static_cast<T>(s->cfg.dt / s->cfg.radius);
     d.tile0 = static_cast<int>(tiles.size());
+    d.target = 0;
+    d.member = m;
     for (int q = 0; q < 6; ++q)
       for (int r = 0; r < 4; ++r)
         for (int k = 0; k < 11; ++k) d.hs[q][r][k] = s->hs.r[q][r][k];
-    tiles.insert(tiles.end(), static_cast<std::size_t>(d.a.nbx) * d.a.nby * 6, m);
+    e->tiles_per_member[m] = 6 * d.a.nbx * d.a.nby;
+    tiles.insert(tiles.end(), static_cast<std::size_t>(e->tiles_per_member[m]), m);
   }
   e->ntiles = static_cast<int>(tiles.size());
-  const std::size_t bd = (md.size() * sizeof(EnsMember<T>) + 255) / 256 * 256;
-  if (hipMalloc(&e->dmem, bd + tiles.size() * sizeof(int)) != hipSuccess) {
+  auto rup = [](std::size_t b) { return (b + 255) / 256 * 256; };
+  const std::size_t bd = rup(md.size() * sizeof(EnsMember<T>)), bt = rup(tiles.size() * sizeof(int));
+  const std::size_t bytes = bd + bt + rup(sizeof(int)) + rup(nm * sizeof(int));
+  if (hipMalloc(&e->dmem, bytes) != hipSuccess) {
     *err = "ensemble: device allocation failed";
     return CCS_ERR_NOMEM;
   }
-  e->d_desc = e->dmem;
-  e->d_tiles = reinterpret_cast<int*>(static_cast<char*>(e->dmem) + bd);
+  char* p = static_cast<char*>(e->dmem);
+  e->d_desc = p;
+  e->d_tiles = reinterpret_cast<int*>(p + bd);
+  e->d_run = reinterpret_cast<int*>(p + bd + bt);
+  e->d_done = reinterpret_cast<int*>(p + bd + bt + rup(sizeof(int)));
+  e->md_nat.assign(reinterpret_cast<const char*>(md.data()), reinterpret_cast<const char*>(md.data() + nm));
+  std::vector<int> done(nm, 0);
+  for (int m = 0; m < nm && done0; ++m) done[m] = static_cast<int>(done0[m] % kRebaseSteps);
   if (hipMemcpy(e->d_desc, md.data(), md.size() * sizeof(EnsMember<T>), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_done, done.data(), nm * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
     *err = "ensemble: table upload failed";
     return CCS_ERR_DEVICE;
   }
@@ -1246,24 +1321,60 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, std::strin
   return CCS_OK;
 }
 
+// Member table and tile map of a run: members ordered by decreasing target
+// (stable, so equal targets keep the member order), tiles following the
+// members.  Synchronous: the stream is drained first, then both tables are
+// replaced.
+template <typename T>
+int ens_set_run_t(EnsPlan* e, hipStream_t st, const int* target, std::string* err) {
+  const int nm = e->nm;
+  std::vector<int> order(nm);
+  for (int m = 0; m < nm; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return target[x] > target[y]; });
+  const auto* nat = reinterpret_cast<const EnsMember<T>*>(e->md_nat.data());
+  std::vector<EnsMember<T>> md(nm);
+  std::vector<int> tiles;
+  tiles.reserve(static_cast<std::size_t>(e->ntiles));
+  for (int k = 0; k < nm; ++k) {
+    const int m = order[k];
+    md[k] = nat[m];
+    md[k].target = target[m];
+    md[k].tile0 = static_cast<int>(tiles.size());
+    tiles.insert(tiles.end(), static_cast<std::size_t>(e->tiles_per_member[m]), k);
+  }
+  if (hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(e->d_desc, md.data(), nm * sizeof(EnsMember<T>), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    *err = "ensemble: run table upload failed";
+    return CCS_ERR_DEVICE;
+  }
+  e->target.assign(target, target + nm);
+  return CCS_OK;
+}
+
 template <typename T>
 void ens_step_t(const EnsPlan* e, hipStream_t st, int scheme) {
+  const int* run = e->d_run;
   if (scheme == CCS_RK3_TVD) {
-    ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0});
-    ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0});
-    ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0});
+    ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0}, run);
+    ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, run);
+    ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0}, run);
   } else {
-    ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0});
-    ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0});
-    ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1});
-    ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0});
+    ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0}, run);
+    ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0}, run);
+    ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1}, run);
+    ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0}, run);
   }
+  const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
+  hipLaunchKernelGGL(ens_rebase<T>, dim3(32, static_cast<unsigned>(e->nm)), dim3(256), 0, st, md, run,
+                     static_cast<const int*>(e->d_done));
+  hipLaunchKernelGGL(ens_tick<T>, dim3(1), dim3(256), 0, st, md, e->nm, e->d_run, e->d_done);
 }
 
 }  // namespace
 
-int ens_plan_create(SplitState* const* mem, int nm, int tile_y, int persistent, int device, EnsPlan** out,
-                    std::string* err) {
+int ens_plan_create(SplitState* const* mem, int nm, int tile_y, int persistent, int device,
+                    const long long* steps_done, EnsPlan** out, std::string* err) {
   if (nm <= 0 || !mem || (tile_y != 8 && tile_y != 16 && tile_y != 32)) {
     *err = "ensemble: invalid plan (members, tile height 8/16/32)";
     return CCS_ERR_INVALID;
@@ -1279,7 +1390,8 @@ int ens_plan_create(SplitState* const* mem, int nm, int tile_y, int persistent, 
   e->tile_y = tile_y;
   e->persistent = persistent ? 1 : 0;
   e->nm = nm;
-  const int rc = fp64 ? ens_build<double>(e, mem, nm, device, err) : ens_build<float>(e, mem, nm, device, err);
+  const int rc = fp64 ? ens_build<double>(e, mem, nm, device, steps_done, err)
+                      : ens_build<float>(e, mem, nm, device, steps_done, err);
   if (rc != CCS_OK) {
     ens_plan_destroy(e);
     return rc;
@@ -1294,20 +1406,36 @@ void ens_plan_destroy(EnsPlan* e) {
   delete e;
 }
 
+int ens_plan_set_run(EnsPlan* e, hipStream_t st, const int* target, std::string* err) {
+  for (int m = 0; m < e->nm; ++m)
+    if (target[m] < 0) {
+      *err = "ensemble: negative step count";
+      return CCS_ERR_INVALID;
+    }
+  if (!std::equal(target, target + e->nm, e->target.begin(), e->target.end())) {
+    const int rc = e->fp64 ? ens_set_run_t<double>(e, st, target, err) : ens_set_run_t<float>(e, st, target, err);
+    if (rc != CCS_OK) return rc;
+  }
+  hipLaunchKernelGGL(ens_run_start, dim3(1), dim3(64), 0, st, e->d_run);
+  return CCS_OK;
+}
+
+int ens_plan_reset_member(EnsPlan* e, hipStream_t st, int member) {
+  const int zero = 0;
+  if (hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(e->d_done + member, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return CCS_ERR_DEVICE;
+  return CCS_OK;
+}
+
 void ens_enqueue_step(const EnsPlan* e, hipStream_t st, int scheme) {
   if (e->fp64) ens_step_t<double>(e, st, scheme);
   else ens_step_t<float>(e, st, scheme);
 }
 
-void ens_enqueue_rebase(const EnsPlan* e, hipStream_t st) {
-  const dim3 grid(512, static_cast<unsigned>(e->nm)), block(256);
-  if (e->fp64) hipLaunchKernelGGL(ens_rebase<double>, grid, block, 0, st, static_cast<const EnsMember<double>*>(e->d_desc));
-  else hipLaunchKernelGGL(ens_rebase<float>, grid, block, 0, st, static_cast<const EnsMember<float>*>(e->d_desc));
-}
-
 void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st) {
-  if (e->fp64) ens_launch<double>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0});
-  else ens_launch<float>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0});
+  if (e->fp64) ens_launch<double>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr);
+  else ens_launch<float>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr);
 }
 
 int ens_plan_grid(const EnsPlan* e) { return e->grid; }

@@ -313,12 +313,32 @@ int ccs_ensemble_diagnostics(ccs_ensemble* e, int member, double* mass,
                              double* energy);
 /* Advance every member nsteps of its own dt. */
 int ccs_ensemble_step(ccs_ensemble* e, int nsteps);
+/* Advance member m by nsteps[m] >= 0 steps of its own dt.  All members step
+ * in the same batched launches; a member that has taken its steps retires
+ * from the tile list (members are ordered by decreasing step count, so the
+ * tiles still stepping are a prefix of the list).  Each member rebases its
+ * geopotential deviation at its own step counts since its own
+ * ccs_ensemble_set_state, as a single run does, so a member ends bit for bit
+ * where ccs_step(nsteps[m]) would. */
+int ccs_ensemble_step_members(ccs_ensemble* e, const int* nsteps);
+/* Advance every member to run_time seconds after its ccs_ensemble_set_state:
+ * member m reaches int(run_time / dt[m]) steps, the integer conversion by
+ * which the reference derives its step count (parameters_mod.F90:149). */
+int ccs_ensemble_run_until(ccs_ensemble* e, double run_time);
+/* Steps member m has taken since its last ccs_ensemble_set_state. */
+int ccs_ensemble_steps_done(const ccs_ensemble* e, int member, long long* steps);
 /* Time tile heights 8/16/32 x {one workgroup per tile, resident grid} on the
  * members' tendency (no state change) and keep the fastest where the config
  * left the choice open; reports the choice and its mean stage time.  Runs by
- * itself on the first ccs_ensemble_step if not called. */
+ * itself on the first step call if not called.  The choice is cached per
+ * device, precision, scheme, halo, multiset of member sizes and config
+ * constraints for the life of the process, and in the file named by the
+ * environment variable CCS_AUTOTUNE_CACHE when it is set; a cached choice is
+ * taken without timing (stage_ms is then the time recorded with it). */
 int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persistent,
                           double* stage_ms);
+/* 1 if the last ccs_ensemble_autotune took its choice from the cache. */
+int ccs_ensemble_autotune_source(const ccs_ensemble* e, int* from_cache);
 /* Wall time of nsteps steps measured with HIP events on the ensemble stream. */
 int ccs_ensemble_time_steps(ccs_ensemble* e, int nsteps, double* total_ms);
 int ccs_ensemble_synchronize(ccs_ensemble* e);

@@ -171,3 +171,58 @@ def test_split_step_grouping_does_not_change_results(ccs, scheme):
         for k in ("phiG", "u", "v"):
             assert np.array_equal(getattr(s, k), getattr(out[0][0], k)), k
         assert d == out[0][1]
+
+
+def test_split_halo_reference_follows_its_source_cell(ccs):
+    # An uploaded state whose halo phiG differs from the interior cell it
+    # copies (here by 1e-3 relative).  Both paths read the halo as uploaded in
+    # the first stage and hold copies of the source cells afterwards
+    # (fill_ghost, ghost_mod.f90:163-170): the split path keeps the halo's
+    # FP64 reference value equal to its source's, so FP64-fast and the FP64
+    # parity path stay together to round-off.
+    mesh, s0, dt, _ = _setup(ccs, n=32, seed=5)
+    sp = s0.copy()
+    halo = np.zeros((6, mesh.P, mesh.P), bool)
+    h, n = mesh.halo, mesh.n
+    halo[:, :, :] = True
+    halo[:, h:h + n, h:h + n] = False
+    halo &= ~mesh.corner_mask()
+    sp.phiG[halo.ravel()] *= 1.0 + 1e-3
+    res = {}
+    for prec in ("fp64", "fp64_fast"):
+        with ccs.DynamicalCore(mesh, dt, precision=prec) as core:
+            core.set_state(sp, with_companions=False)
+            core.step(5)
+            res[prec] = core.get_state()
+    mask = ~mesh.corner_mask().ravel()
+    for k in ("phiG", "u", "v"):
+        a, b = getattr(res["fp64_fast"], k)[mask], getattr(res["fp64"], k)[mask]
+        e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        print(f"  {k}: max rel diff {e:.3e}", flush=True)
+        assert e <= 1e-11
+
+
+@pytest.mark.parametrize("n,hours", [(48, 24.0), (96, 12.0)])
+def test_case2_steady_flow_fp32_vs_fp64(ccs, n, hours):
+    # Williamson case 2 (test_case_mod.F90:61-95): a steady zonal flow, so the
+    # rounding of the FP32 path is not amplified by developing structure.
+    mesh = ccs.Mesh(n, halo=3, radius=6.371e6)
+    s0 = ccs.init_case(mesh, case_num=2)
+    dt, nsteps = ccs.cfl_time_step(mesh, s0, 0.5, hours * 3600.0)
+    out = {}
+    for name, prec, st0 in (("fp32", "fp32", s0), ("fp64", "fp64", s0),
+                            ("floor", "fp64", ccs.perturb_state(mesh, s0, 2.0 ** -24))):
+        with ccs.DynamicalCore(mesh, dt, precision=prec) as core:
+            core.set_state(st0)
+            m0, _ = core.diagnostics()
+            core.step(nsteps)
+            m1, _ = core.diagnostics()
+            out[name] = (core.get_state(), (m1 - m0) / m0)
+    area = mesh.interior(mesh.areaCell)
+    ref = mesh.interior(out["fp64"][0].phi) / G
+    e32 = ccs.error_norms(mesh.interior(out["fp32"][0].phi) / G, ref, area)
+    efl = ccs.error_norms(mesh.interior(out["floor"][0].phi) / G, ref, area)
+    print(f"case 2 C{n}, {hours} h ({nsteps} steps): L2(h) fp32 {e32[1]:.3e} (floor {efl[1]:.3e}), "
+          f"Linf {e32[2]:.3e}; mass drift fp32 {out['fp32'][1]:.3e}", flush=True)
+    assert e32[1] <= 4.0 * efl[1] + 1e-9
+    assert abs(out["fp32"][1]) <= 1e-10
