@@ -281,7 +281,10 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
       ccs_destroy(hd);
       return CCS_ERR_NOMEM;
     }
-    if (hipMemset(hd->dmem, 0, total * sizeof(double)) != hipSuccess) {
+    // cleared on the handle's stream, and waited for: a memset on the null
+    // stream is not ordered before the uploads on a non-blocking stream
+    if (hipMemsetAsync(hd->dmem, 0, total * sizeof(double), hd->stream) != hipSuccess ||
+        hipStreamSynchronize(hd->stream) != hipSuccess) {
       ccs_destroy(hd);
       return CCS_ERR_DEVICE;
     }
@@ -304,7 +307,7 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
       return CCS_ERR_INVALID;
     }
   } else {
-    rc = split_create(*cfg, hd->hs, &hd->split, &hd->err);
+    rc = split_create(*cfg, hd->hs, hd->stream, &hd->split, &hd->err);
     if (rc != CCS_OK) {
       ccs_destroy(hd);
       return rc;

@@ -14,7 +14,7 @@ namespace ccs {
 
 struct SplitState;
 
-int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out, std::string* err);
+int split_create(const ccs_config& cfg, const HaloScatter& hs, hipStream_t st, SplitState** out, std::string* err);
 void split_destroy(SplitState* s);
 // T: the 15 host-side SoA tables built by ccs_set_mesh (sqrtG, IG11, IG12,
 // IG21, IG22, phis, f, M11, M12, M21, M22, N11, N12, N21, N22), padded.

@@ -921,7 +921,7 @@ void download_to_double(hipStream_t st, const void* src, std::size_t cnt, std::v
 
 }  // namespace
 
-int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out, std::string* err) {
+int split_create(const ccs_config& cfg, const HaloScatter& hs, hipStream_t st, SplitState** out, std::string* err) {
   SplitState* s = new SplitState();
   s->cfg = cfg;
   s->hs = hs;
@@ -959,8 +959,10 @@ int split_create(const ccs_config& cfg, const HaloScatter& hs, SplitState** out,
     delete s;
     return CCS_ERR_NOMEM;
   }
-  if (hipMemset(s->dmem, 0, bytes) != hipSuccess) {
-    *err = "split_create: hipMemset failed";
+  // cleared on the handle's stream, and waited for: a memset on the null
+  // stream is not ordered before the uploads on a non-blocking stream
+  if (hipMemsetAsync(s->dmem, 0, bytes, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    *err = "split_create: clearing the device buffers failed";
     (void)hipFree(s->dmem);
     delete s;
     return CCS_ERR_DEVICE;

@@ -226,3 +226,29 @@ def test_case2_steady_flow_fp32_vs_fp64(ccs, n, hours):
           f"Linf {e32[2]:.3e}; mass drift fp32 {out['fp32'][1]:.3e}", flush=True)
     assert e32[1] <= 4.0 * efl[1] + 1e-9
     assert abs(out["fp32"][1]) <= 1e-10
+
+
+def test_handle_created_while_another_run_is_in_flight(ccs):
+    # A handle is created, cleared and uploaded while a long run of another
+    # handle still occupies the GPU (its steps are asynchronous): the new
+    # handle's buffers must be cleared before its uploads land, whatever the
+    # other streams are doing, so its run equals the same run on an idle GPU.
+    mesh, s0, dt, _ = _setup(ccs, n=24, seed=21)
+    big, sb, dtb, _ = _setup(ccs, n=360, seed=22, hours=6.0)
+    with ccs.DynamicalCore(mesh, dt, precision="fp32") as core:
+        core.set_state(s0)
+        core.step(30)
+        idle = core.get_state()
+    with ccs.DynamicalCore(big, dtb, precision="fp32") as busy:
+        busy.set_state(sb)
+        busy.step(400)  # not synchronised: still running below
+        for prec in ("fp32", "fp64"):
+            with ccs.DynamicalCore(mesh, dt, precision=prec) as core:
+                core.set_state(s0)
+                core.step(30)
+                s = core.get_state()
+                assert np.all(np.isfinite(mesh.interior(s.phi)))
+                if prec == "fp32":
+                    for k in ("phiG", "u", "v"):
+                        assert np.array_equal(getattr(s, k), getattr(idle, k)), k
+        busy.synchronize()
