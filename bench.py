@@ -9,6 +9,11 @@ all six panels (3 fused stage kernels).  Inputs are resident in HBM before the
 timed region.  The split-precision path (FP32 state) is the headline `value`;
 the FP64 parity path is timed beside it at the same configuration.
 
+Before the W warmup steps every timed path runs untimed steps for 0.3 s
+(`settle_steps` in the line): the GPU's clocks ramp up over the first ~0.1 s
+of sustained work, and a 20-step timed region would otherwise be measured on
+a GPU that is still ramping.
+
 Prints one JSON line.  `python bench.py --gpus N --steps K --warmup W`.
 """
 from __future__ import annotations
@@ -90,10 +95,18 @@ def copy_calibration(mib, reps):
     print(f"copied {mib} MiB x {reps}: read {mib} MiB and wrote {mib} MiB per dispatch", flush=True)
 
 
-def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno"):
+def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno", settle_s=0.3):
     core = ccs.DynamicalCore(mesh, dt, precision=precision, use_graph=True, spatial_op=spatial_op)
     core.set_state(s0)
     core.step(1)          # first step reads the uploaded companions
+    # untimed steps until the GPU has run for settle_s (its clocks ramp up
+    # over the first ~0.1 s of sustained work), then the warmup steps
+    settle = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < settle_s:
+        core.step(16)
+        core.synchronize()
+        settle += 16
     core.step(warmup)
     core.synchronize()
     t0 = time.perf_counter()
@@ -103,7 +116,8 @@ def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno"):
     ev_total, _ = core.time_steps(steps)            # HIP events around the graph replays
     nk = min(steps, 60)
     _, kern_ms = core.time_steps(nk, kernel_timing=True)  # events around every stage kernel
-    core.advanced = 1 + warmup + 2 * steps + 2 * nk  # time_steps(kernel_timing) runs two passes
+    core.advanced = 1 + settle + warmup + 2 * steps + 2 * nk  # time_steps(kernel_timing) runs two passes
+    core.settle_steps = settle
     return core, wall, ev_total, kern_ms
 
 
@@ -286,6 +300,7 @@ def main():
                      "limiter": "hbm" if args.spatial_op.startswith("centred") else "valu",
                      "steps_per_s_over_stage_kernels": (3 * kern_ms) / (wall * 1e3 / args.steps)},
         "event_ms_per_step": ev_total / args.steps,
+        "settle_steps": core.settle_steps,
     }
     prof = os.path.join(ROOT, "profiles", f"traffic_c{args.n}_{headline}.json")
     if os.path.exists(prof) and args.spatial_op == "weno":

@@ -17,7 +17,8 @@ zonal flow of test_case_mod.F90:61-95 (--case 2), whose exact solution does
 not change in time, so rounding errors are not amplified by developing
 structure.
 
-Every run records, at --snapshots evenly spaced times, the normalised L2 of h
+Every run records, at --snapshots evenly spaced times (or at the simulated
+--hours given), the normalised L2 of h
 against the FP64 run at the same time; at the end the normalised L1/L2/Linf of
 h, zonal u and meridional v (area-weighted, as doc/orderCheck.m:55-74).  One
 JSON line is printed at the end; progress lines go to stderr.  `--only fp32`
@@ -88,7 +89,8 @@ def main():
                     help="FP64 reference: the parity kernel or the split kernel instantiated in FP64")
     ap.add_argument("--perturb", default="24",
                     help="comma-separated k: FP64 runs from the state perturbed by 2^-k (empty: none)")
-    ap.add_argument("--snapshots", type=int, default=0, help="L2(h) against FP64 at this many times")
+    ap.add_argument("--snapshots", type=int, default=0, help="L2(h) against FP64 at this many evenly spaced times")
+    ap.add_argument("--hours", default="", help="comma-separated simulated hours for the L2(h) series instead")
     ap.add_argument("--chunk", type=int, default=4000)
     ap.add_argument("--only", choices=["fp32"], default=None,
                     help="run only the FP32 path (drift, stability and throughput; no error norms)")
@@ -108,6 +110,8 @@ def main():
         print(json.dumps(out), flush=True)
         return
     marks = [round(nsteps * (k + 1) / args.snapshots) for k in range(args.snapshots)] if args.snapshots else []
+    if args.hours:
+        marks = sorted({min(nsteps, max(1, round(float(x) * 3600.0 / dt))) for x in args.hours.split(",")})
     ref, ref_h, ref_info = _run(mesh, s0, dt, nsteps, args.reference, True, args.chunk, args.reference, marks)
     f32, f32_l2, f32_info = _run(mesh, s0, dt, nsteps, "fp32", True, args.chunk, "fp32", marks, ref_h)
     out = {"n": args.n, "case": args.case, "days": args.days, "steps": nsteps, "dt_s": dt,

@@ -103,6 +103,7 @@ struct SplitArgs {
   const T* tacc[9];
   int n, h;
   int nbx, nby;        // tiles per panel along x and y
+  unsigned mbx, mby;   // division by nbx, nby: q = hi32((2 t) mb), exact for t < 2^20 (tile_magic)
   T rdx;               // 1 / dx
   T dts, incs;         // dt / a, RK4 increment / a
   // WENO-Z epsilons of Fx/Fy, E (windows scaled by 1/2: times 1/16) and of
@@ -126,13 +127,16 @@ struct SplitArgs {
 //
 // Scaled window (SC = 2 or 4): the candidates, hence the returned face
 // values, are 1/SC of the true deviations, so a flux share 0.5 (F + lam P)
-// becomes one FMA of the scaled values.  The indicators come out multiplied
-// by (1/(2 SC))^2 and the caller passes epsilon multiplied by the same power
-// of two, so tau5 / (eps + beta_k), and with it every weight, is that of the
-// unscaled window.
+// becomes one FMA of the scaled values.  The window's inputs come scaled by
+// 1/(4 SC) (the phase-1 fields are stored that way): then the first
+// differences b_k / (4 SC) are a_k - 2 d_k and d1 - d3 of the scaled inputs,
+// one operation each.  The indicators come out multiplied by (1/(2 SC))^2 and
+// the caller passes epsilon multiplied by the same power of two, so
+// tau5 / (eps + beta_k), and with it every weight, is that of the unscaled
+// window.
 template <typename T, int SC>
 struct WenoWin {
-  T a1, a2, a3;     // second differences (relative to q2)
+  T a1, a2, a3;     // second differences (relative to q2), of the scaled inputs
   T q1, q2, q3;     // first differences b_k / (4 SC)
   // WENO-Z weights 10 d_k (1 + tau5 / (eps + beta_k)), 10 d = (1, 6, 3): the
   // right face takes (u1, u2, u3), the mirrored left face (v3, u2, v1), with
@@ -145,7 +149,8 @@ struct WenoWin {
 // the differences q_k - q2 (exact in FP32 for neighbouring values of one
 // sign, by Sterbenz' lemma) and the reconstructions are returned as
 // deviations from q2.  The weights only see differences, so they are those
-// of the absolute form; the absolute face value is q2 + deviation.
+// of the absolute form; the absolute face value is q2 + deviation.  q0..q4
+// are the window's values times 1/(4 SC).
 template <typename T, int SC>
 __device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T q4, T eps_scaled) {
 #pragma clang fp contract(fast)
@@ -154,13 +159,13 @@ __device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T 
   w.a1 = T(-2) * d1 + d0;
   w.a2 = d1 + d3;
   w.a3 = T(-2) * d3 + d4;
-  // b1/2 = a1/2 - d1, b2/2 = (d1 - d3)/2, b3/2 = a3/2 - d3, each times 1/(2 SC)
-  constexpr T kh = T(1) / T(2 * SC), kq = T(1) / T(4 * SC);
-  w.q1 = kq * w.a1 - kh * d1;
-  w.q2 = kq * (d1 - d3);
-  w.q3 = kq * w.a3 - kh * d3;
-  // beta_k / (2 SC)^2 = 13/12 a_k^2 / (2 SC)^2 + (b_k / (4 SC))^2
-  const T k13 = (T(13) / T(12)) * (T(1) / T(4 * SC * SC));
+  // b1 = a1 - 2 d1, b2 = d1 - d3, b3 = a3 - 2 d3 (times 1/(4 SC) with the inputs)
+  w.q1 = T(-2) * d1 + w.a1;
+  w.q2 = d1 - d3;
+  w.q3 = T(-2) * d3 + w.a3;
+  // beta_k / (2 SC)^2 = 13/12 a_k^2 / (2 SC)^2 + (b_k / (4 SC))^2; with the
+  // scaled a_k that is 13/3 a_k^2 + q_k^2
+  const T k13 = T(13) / T(3);
   const T be1 = (k13 * w.a1) * w.a1 + w.q1 * w.q1;
   const T be2 = (k13 * w.a2) * w.a2 + w.q2 * w.q2;
   const T be3 = (k13 * w.a3) * w.a3 + w.q3 * w.q3;
@@ -212,7 +217,7 @@ __device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T w1, T w2, T w3) {
 template <typename T, int SC>
 __device__ __forceinline__ void weno_faces(const WenoWin<T, SC>& w, T& right, T& left) {
 #pragma clang fp contract(fast)
-  constexpr T kc = T(1) / T(12 * SC);
+  constexpr T kc = T(1) / T(3);  // a_k / (12 SC) of the inputs scaled by 1/(4 SC)
   T r1 = w.u1, r2 = w.u2, r3 = w.u3, l1 = w.v3, l2 = w.u2, l3 = w.v1;
   if ((w.t40 <= w.mb) != (w.t41 <= w.mb)) {
     // an empty volatile asm keeps this a branch (skipped by whole
@@ -229,7 +234,7 @@ __device__ __forceinline__ void weno_faces(const WenoWin<T, SC>& w, T& right, T&
 template <typename T, int SC>
 __device__ __forceinline__ T weno_side(const WenoWin<T, SC>& w, bool left) {
 #pragma clang fp contract(fast)
-  constexpr T kc = T(1) / T(12 * SC);
+  constexpr T kc = T(1) / T(3);
   const T s1 = left ? w.a3 * kc + w.q3 : w.a1 * kc + w.q1;
   const T s2 = w.a2 * kc + (left ? w.q2 : -w.q2);
   const T s3 = left ? w.a1 * kc - w.q1 : w.a3 * kc - w.q3;
@@ -299,59 +304,100 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
 
   const int n = a.n, h = a.h;
   const unsigned P = static_cast<unsigned>(n + 2 * h);
+  // scale of the stored E, Fx, Fy (kF) and phitG (kP): 1/(4 SC) of the
+  // windows that read them (SC = 2 and 4); the centred operators store them
+  // unscaled
+  constexpr T kF = kWeno ? T(1) / T(8) : T(1);
+  constexpr T kP = kWeno ? T(1) / T(16) : T(1);
   // padded index of the extended tile's (0, 0) cell (i0 - 3, j0 - 3); h >= 3
   const unsigned gt = static_cast<unsigned>(p) * P * P + static_cast<unsigned>(j0 - 3 + h) * P +
                       static_cast<unsigned>(i0 - 3 + h);
 
-  // ---- phase 1
-  constexpr int NX = EX * TY;
-  constexpr int NCROSS = NX + 6 * TX;
-  for (int e = tid; e < NCROSS; e += NT) {
-    int li, lj;
-    if (e < NX) {
-      li = e % EX; lj = 3 + e / EX;
-    } else {
-      const int e2 = e - NX;
-      li = 3 + e2 % TX;
-      const int r = e2 / TX;
-      lj = (r < 3) ? r : TY + r;
-    }
+  // ---- phase 1: pointwise terms of the extended tile -> LDS.  Every
+  // thread takes its own cell; the 6 TX cells of the y-border rows and the
+  // 6 TY cells of the x-border columns go to the highest threads, so that
+  // they do not fall on the waves that evaluate the boundary windows of
+  // phase 2 (the lowest threads).  Both cells' loads are issued before
+  // either is used (a cell outside the padded panel, or in a halo corner,
+  // which is never read, loads from the tile origin and is not stored), so a
+  // thread with two cells waits for memory once.
+  struct Raw {
+    T pr, dp, u, v, sg, ps;
+    int l;       // extended-tile index, -1: not stored
+  };
+  auto fetch = [&](const int li, const int lj, const bool want) {
     const int i = i0 + li - 3, j = j0 + lj - 3;
-    if (i < -h || i >= n + h || j < -h || j >= n + h) continue;
-    if ((i < 0 || i >= n) && (j < 0 || j >= n)) continue;
-    const unsigned go = (gt + __umul24(static_cast<unsigned>(lj), P) + static_cast<unsigned>(li)) * W;
-    const T Pv = ldo(st.pref, go) + ldo(a.in.phiG, go);
-    const T U = ldo(a.in.u, go), V = ldo(a.in.v, go);
-    const T sg = ldo(st.sg, go);
-    const T rg = rcp(sg);
-    const T Uc = U * rg, Vc = V * rg;
-    const T phit = Pv * rg + ldo(st.phis, go);
-    const int l = lj * EX + li;
-    sE[l] = phit + T(0.5) * (U * Uc + V * Vc);
+    const bool ok = want && i >= -h && i < n + h && j >= -h && j < n + h && !((i < 0 || i >= n) && (j < 0 || j >= n));
+    const unsigned go = ok ? (gt + __umul24(static_cast<unsigned>(lj), P) + static_cast<unsigned>(li)) * W
+                           : (gt + 3u * P + 3u) * W;
+    Raw r;
+    r.pr = ldo(st.pref, go);
+    r.dp = ldo(a.in.phiG, go);
+    r.u = ldo(a.in.u, go);
+    r.v = ldo(a.in.v, go);
+    r.sg = ldo(st.sg, go);
+    r.ps = ldo(st.phis, go);
+    r.l = ok ? lj * EX + li : -1;
+    return r;
+  };
+  auto pointwise = [&](const Raw& r) {
+    if (r.l < 0) return;
+    const T Pv = r.pr + r.dp;
+    const T U = r.u, V = r.v;
+    const T rg = rcp(r.sg);
+    // for the WENO windows E, Fx, Fy are stored times kF = 1/8 and phitG
+    // times kP = 1/16 (see weno_window; the surface geopotential static is
+    // uploaded times kF): the contravariant winds carry the factor kF
+    const T rgc = rg * kF;
+    const T Uc = U * rgc, Vc = V * rgc;
+    const T phitc = Pv * rgc + r.ps;
+    const int l = r.l;
+    sE[l] = phitc + T(0.5) * (U * Uc + V * Vc);
     sFx[l] = Pv * Uc;
     sFy[l] = Pv * Vc;
-    sPt[l] = phit * sg;
+    sPt[l] = phitc * (r.sg * (kP / kF));
     sRg[l] = rg;
     sU[l] = U;
     sV[l] = V;
+  };
+  {
+    constexpr int NYB = 6 * TX, NB = NYB + 6 * TY;
+    static_assert(NB <= NT, "one pass of border cells");
+    const int k = tid - (NT - NB);
+    int bli, blj;
+    if (k < NYB) {
+      const int r = k / TX;
+      bli = 3 + k % TX;
+      blj = r < 3 ? r : TY + r;
+    } else {
+      const int kk = k - NYB, c = kk % 6;
+      bli = c < 3 ? c : TX + c;
+      blj = 3 + kk / 6;
+    }
+    const Raw own = fetch((tid % TX) + 3, (tid / TX) + 3, true);
+    const Raw border = fetch(bli, blj, k >= 0);
+    pointwise(own);
+    pointwise(border);
   }
   __syncthreads();
 
   // Twice the Rusanov speed at the x-face between extended-tile cells l and
   // l + 1, and at the y-face between l and l + EX
   // (spatial_operators_mod.F90:109-116, 134-141, on a conformal grid): the
-  // face means are left as sums, 2 |ub| + sqrt(4 gb pb) exactly.
+  // face means are left as sums, 2 |ub| + sqrt(4 gb pb) exactly (the stored
+  // phitG carries kP = 1/16, so its square root a factor 1/4).
+  constexpr T kRootP = T(4);
   auto lam_x = [&](int l) {
     const T r0 = sRg[l], r1 = sRg[l + 1];
     const T ub = sU[l] * r0 + sU[l + 1] * r1;
     const T pb = sPt[l] * r0 + sPt[l + 1] * r1;
-    return fabs(ub) + fsqrt((r0 + r1) * pb);
+    return fsqrt((r0 + r1) * pb) * kRootP + fabs(ub);
   };
   auto lam_y = [&](int l) {
     const T r0 = sRg[l], r1 = sRg[l + EX];
     const T vb = sV[l] * r0 + sV[l + EX] * r1;
     const T pb = sPt[l] * r0 + sPt[l + EX] * r1;
-    return fabs(vb) + fsqrt((r0 + r1) * pb);
+    return fsqrt((r0 + r1) * pb) * kRootP + fabs(vb);
   };
 
   // ---- phase 2: WENO windows and face shares.  Thread (tx, ty) owns the
@@ -365,6 +411,33 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   // centre terms from differences of neighbouring centre values.  Own shares
   // and Rusanov speeds stay in registers.
   const int tx = tid % TX, ty = tid / TX;
+  // phase-3 inputs of the thread's own cell: the Coriolis static, the base
+  // state and, where the stage combines it, the stage input.  The WENO path
+  // fetches them before the barrier that ends phase 2, so their memory
+  // latency overlaps the wait (a thread whose cell lies outside the panel
+  // fetches the tile origin and stores nothing).
+  struct Pre {
+    T fs, b0, b1, b2, c0, c1, c2;
+  };
+  auto preload = [&]() {
+    const bool ok = (i0 + tx < n) && (j0 + ty < n);
+    const unsigned g = (gt + __umul24(static_cast<unsigned>(ok ? ty + 3 : 3), P) +
+                        static_cast<unsigned>(ok ? tx + 3 : 3)) * W;
+    Pre q{};
+    q.fs = ldo(st.fs, g);
+    if (KIND != kTendOnly) {
+      q.b0 = ldo(a.base.phiG, g);
+      q.b1 = ldo(a.base.u, g);
+      q.b2 = ldo(a.base.v, g);
+    }
+    if (KIND == kRK3s2 || KIND == kRK3s3) {
+      q.c0 = ldo(a.in.phiG, g);
+      q.c1 = ldo(a.in.u, g);
+      q.c2 = ldo(a.in.v, g);
+    }
+    return q;
+  };
+  Pre pre{};
   T fr, fl, er, el, fu, fd, eu, ed, lr, ll, lu, ld;
   if constexpr (kWeno) {
     const int l = (ty + 3) * EX + tx + 3;
@@ -410,50 +483,57 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
     // windows centred one cell outside the tile, which give only the share
     // of the tile's boundary face: x: centres -1 (right face -> x-face 0) and
     // TX (left face -> x-face TX) of each row; y: centres -1 and TY of each
-    // column.  One lane per window centre, all three variables.
+    // column.  One lane per window centre and role: the lowest NEXTRA
+    // threads evaluate the phitG and flux windows and the Rusanov speed (the
+    // flux share), the NEXTRA threads from the next wavefront boundary on the
+    // E window, so that the two roles fall on different wavefronts.
     constexpr int NXE = 2 * TY, NEXTRA = 2 * TY + 2 * TX;
-    static_assert(NEXTRA <= NT, "one pass of extra windows");
-    if (tid < NEXTRA) {
-      int l0, stride, lf;
+    constexpr int E0 = (NEXTRA + 63) / 64 * 64;
+    static_assert(E0 + NEXTRA <= NT, "one pass of extra windows");
+    const bool eRole = tid >= E0 && tid < E0 + NEXTRA;
+    if (tid < NEXTRA || eRole) {
+      const int q = eRole ? tid - E0 : tid;
+      int l0, stride, lf, f;
       bool left;
-      T* oF;
-      T* oE;
-      if (tid < NXE) {
-        const int row = tid >> 1;
-        left = (tid & 1) != 0;
+      if (q < NXE) {
+        const int row = q >> 1;
+        left = (q & 1) != 0;
         l0 = (row + 3) * EX + (left ? TX : -1) + 3;
         stride = 1;
-        const int f = row * (TX + 1) + (left ? TX : 0);
-        oF = left ? xmF + f : xpF + f;
-        oE = left ? xmE + f : xpE + f;
+        f = row * (TX + 1) + (left ? TX : 0);
         lf = left ? l0 - 1 : l0;  // cell left of the face
       } else {
-        const int k2 = tid - NXE;
+        const int k2 = q - NXE;
         const int col = k2 % TX;
         left = k2 >= TX;
         l0 = ((left ? TY : -1) + 3) * EX + col + 3;
         stride = EX;
-        const int f = (left ? TY * TX : 0) + col;
-        oF = left ? ymF + f : ypF + f;
-        oE = left ? ymE + f : ypE + f;
+        f = (left ? TY * TX : 0) + col;
         lf = left ? l0 - EX : l0;  // cell below the face
       }
-      const T* sF = (tid < NXE) ? sFx : sFy;
-      const auto wp = weno_window<T, 4>(sPt[l0 - 2 * stride], sPt[l0 - stride], sPt[l0], sPt[l0 + stride],
-                                        sPt[l0 + 2 * stride], a.epsP);
-      const auto wf = weno_window<T, 2>(sF[l0 - 2 * stride], sF[l0 - stride], sF[l0], sF[l0 + stride],
-                                        sF[l0 + 2 * stride], a.epsF);
-      const auto we = weno_window<T, 2>(sE[l0 - 2 * stride], sE[l0 - stride], sE[l0], sE[l0 + stride],
-                                        sE[l0 + 2 * stride], a.epsE);
-      const T lam = (tid < NXE) ? lam_x(lf) : lam_y(lf);  // twice the speed
-      const T vp = weno_side(wp, left), vf = weno_side(wf, left), ve = weno_side(we, left);
-      *oF = left ? vf - lam * vp : vf + lam * vp;
-      *oE = ve;
+      if (eRole) {
+        const auto we = weno_window<T, 2>(sE[l0 - 2 * stride], sE[l0 - stride], sE[l0], sE[l0 + stride],
+                                          sE[l0 + 2 * stride], a.epsE);
+        T* oE = (q < NXE) ? (left ? xmE : xpE) : (left ? ymE : ypE);
+        oE[f] = weno_side(we, left);
+      } else {
+        const T* sF = (q < NXE) ? sFx : sFy;
+        const auto wp = weno_window<T, 4>(sPt[l0 - 2 * stride], sPt[l0 - stride], sPt[l0], sPt[l0 + stride],
+                                          sPt[l0 + 2 * stride], a.epsP);
+        const auto wf = weno_window<T, 2>(sF[l0 - 2 * stride], sF[l0 - stride], sF[l0], sF[l0 + stride],
+                                          sF[l0 + 2 * stride], a.epsF);
+        const T lam = (q < NXE) ? lam_x(lf) : lam_y(lf);  // twice the speed
+        const T vp = weno_side(wp, left), vf = weno_side(wf, left);
+        T* oF = (q < NXE) ? (left ? xmF : xpF) : (left ? ymF : ypF);
+        oF[f] = left ? vf - lam * vp : vf + lam * vp;
+      }
     }
+    pre = preload();
     __syncthreads();
   }
 
   // ---- phase 3: cells
+  if constexpr (!kWeno) pre = preload();
   const int i = i0 + tx, j = j0 + ty;
   if (i >= n || j >= n) return;
   const T rdx = a.rdx;
@@ -466,13 +546,15 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   if constexpr (kWeno) {
     const int fx = ty * (TX + 1) + tx, fy = ty * TX + tx;
     const T pc = sPt[lc];
-    // (lr, ll, lu, ld hold twice the Rusanov speeds)
-    const T cx = T(0.5) * (sFx[lc + 1] - sFx[lc - 1]) - T(0.25) * (lr * (sPt[lc + 1] - pc) - ll * (pc - sPt[lc - 1]));
-    const T cy = T(0.5) * (sFy[lc + EX] - sFy[lc - EX]) - T(0.25) * (lu * (sPt[lc + EX] - pc) - ld * (pc - sPt[lc - EX]));
+    // (lr, ll, lu, ld hold twice the Rusanov speeds; the stored fields carry
+    // kF = 1/8 and kP = 1/16, so 0.5 dF and 0.25 lam dP both become 4 times
+    // the stored differences)
+    const T cx = T(4) * ((sFx[lc + 1] - sFx[lc - 1]) - (lr * (sPt[lc + 1] - pc) - ll * (pc - sPt[lc - 1])));
+    const T cy = T(4) * ((sFy[lc + EX] - sFy[lc - EX]) - (lu * (sPt[lc + EX] - pc) - ld * (pc - sPt[lc - EX])));
     dFx = (cx + ((fr + xmF[fx + 1]) - (xpF[fx] + fl))) * rdx;
-    dEx = (T(0.5) * (sE[lc + 1] - sE[lc - 1]) + ((er + xmE[fx + 1]) - (xpE[fx] + el))) * rdx;
+    dEx = (T(4) * (sE[lc + 1] - sE[lc - 1]) + ((er + xmE[fx + 1]) - (xpE[fx] + el))) * rdx;
     dFy = (cy + ((fu + ymF[fy + TX]) - (ypF[fy] + fd))) * rdx;
-    dEy = (T(0.5) * (sE[lc + EX] - sE[lc - EX]) + ((eu + ymE[fy + TX]) - (ypE[fy] + ed))) * rdx;
+    dEy = (T(4) * (sE[lc + EX] - sE[lc - EX]) + ((eu + ymE[fy + TX]) - (ypE[fy] + ed))) * rdx;
     dvdx = d6f(sV[lc - 3], sV[lc - 2], sV[lc - 1], sV[lc + 1], sV[lc + 2], sV[lc + 3], rdx);
     dudy = d6f(sU[lc - 3 * EX], sU[lc - 2 * EX], sU[lc - EX], sU[lc + EX], sU[lc + 2 * EX], sU[lc + 3 * EX], rdx);
   } else {
@@ -494,7 +576,7 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
     dudy = cd(sU, EX);
   }
   const unsigned go = (gt + __umul24(static_cast<unsigned>(ty + 3), P) + static_cast<unsigned>(tx + 3)) * W;
-  const T zeta = dvdx - dudy + ldo(st.fs, go);
+  const T zeta = dvdx - dudy + pre.fs;
   const T tp = -(dFx + dFy);
   const T rgc = sRg[lc];
   const T tu = -dEx + zeta * (sV[lc] * rgc);
@@ -511,34 +593,34 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   }
   T pn, un, vn;
   if (KIND == kRK3s1) {
-    pn = ldo(a.base.phiG, go) + a.dts * tp;
-    un = ldo(a.base.u, go) + a.dts * tu;
-    vn = ldo(a.base.v, go) + a.dts * tv;
+    pn = pre.b0 + a.dts * tp;
+    un = pre.b1 + a.dts * tu;
+    vn = pre.b2 + a.dts * tv;
   } else if (KIND == kRK3s2) {
     const T w = T(0.25) * a.dts;
-    pn = T(0.75) * ldo(a.base.phiG, go) + T(0.25) * ldo(a.in.phiG, go) + w * tp;
-    un = T(0.75) * ldo(a.base.u, go) + T(0.25) * ldo(a.in.u, go) + w * tu;
-    vn = T(0.75) * ldo(a.base.v, go) + T(0.25) * ldo(a.in.v, go) + w * tv;
+    pn = T(0.75) * pre.b0 + T(0.25) * pre.c0 + w * tp;
+    un = T(0.75) * pre.b1 + T(0.25) * pre.c1 + w * tu;
+    vn = T(0.75) * pre.b2 + T(0.25) * pre.c2 + w * tv;
   } else if (KIND == kRK3s3) {
     const T c3 = T(1) / T(3), c23 = T(2) / T(3);
     const T w = c23 * a.dts;
-    pn = c3 * ldo(a.base.phiG, go) + c23 * ldo(a.in.phiG, go) + w * tp;
-    un = c3 * ldo(a.base.u, go) + c23 * ldo(a.in.u, go) + w * tu;
-    vn = c3 * ldo(a.base.v, go) + c23 * ldo(a.in.v, go) + w * tv;
+    pn = c3 * pre.b0 + c23 * pre.c0 + w * tp;
+    un = c3 * pre.b1 + c23 * pre.c1 + w * tu;
+    vn = c3 * pre.b2 + c23 * pre.c2 + w * tv;
   } else if (KIND == kRK4a) {
     sto(a.tend[0], io, tp);
     sto(a.tend[1], io, tu);
     sto(a.tend[2], io, tv);
-    pn = ldo(a.base.phiG, go) + a.incs * tp;
-    un = ldo(a.base.u, go) + a.incs * tu;
-    vn = ldo(a.base.v, go) + a.incs * tv;
+    pn = pre.b0 + a.incs * tp;
+    un = pre.b1 + a.incs * tu;
+    vn = pre.b2 + a.incs * tv;
   } else {
     const T c6 = T(1) / T(6);
-    pn = ldo(a.base.phiG, go) +
+    pn = pre.b0 +
          a.dts * ((ldo(a.tacc[0], io) + T(2) * (ldo(a.tacc[3], io) + ldo(a.tacc[6], io)) + tp) * c6);
-    un = ldo(a.base.u, go) +
+    un = pre.b1 +
          a.dts * ((ldo(a.tacc[1], io) + T(2) * (ldo(a.tacc[4], io) + ldo(a.tacc[7], io)) + tu) * c6);
-    vn = ldo(a.base.v, go) +
+    vn = pre.b2 +
          a.dts * ((ldo(a.tacc[2], io) + T(2) * (ldo(a.tacc[5], io) + ldo(a.tacc[8], io)) + tv) * c6);
   }
   sto(a.out.phiG, go, pn);
@@ -566,8 +648,8 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
 template <typename T, int KIND, int TY, int OP>
 __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
   const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
-  const unsigned tq = t / static_cast<unsigned>(a.nbx);
-  const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
+  const unsigned tq = __umulhi(t << 1, a.mbx);
+  const int p = static_cast<int>(__umulhi(tq << 1, a.mby));
   const int i0 = static_cast<int>(t - tq * a.nbx) * TX;
   const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
   stage_tile<T, KIND, TY, OP>(a, st, hs.r[p], p, i0, j0, static_cast<int>(threadIdx.x));
@@ -599,11 +681,10 @@ struct EnsStage {
   int inc_full; // RK4: 1 -> dt/a increment, 0 -> dt/(2a)
 };
 
-// Step index within the current run (device counter, advanced by ens_tick),
-// or -1 where every member steps (the tendency launch of the autotuner).
-__device__ __forceinline__ int run_step(const int* run_c) {
-  return run_c ? __hip_atomic_load(run_c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
-}
+// Step index within the current run (device counter, advanced by ens_tick
+// in an earlier launch of the stream, so a plain uniform load sees it), or -1
+// where every member steps (the tendency launch of the autotuner).
+__device__ __forceinline__ int run_step(const int* __restrict__ run_c) { return run_c ? *run_c : -1; }
 
 // One tile of the global list; false when its member has taken its steps
 // (members are ordered by decreasing step count, so every later tile of the
@@ -694,16 +775,16 @@ __global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict
 // thread 0 advances it).
 template <typename T>
 __global__ void __launch_bounds__(256) ens_tick(const EnsMember<T>* __restrict__ md, int nm, int* run_c, int* done) {
-  const int c = run_step(run_c);
+  const int c = *run_c;
   for (int k = threadIdx.x; k < nm; k += 256)
     if (c < md[k].target) done[md[k].member] = (done[md[k].member] + 1) % kRebaseSteps;
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(run_c, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) *run_c = c + 1;
 }
 
 // Start of a run: step index 0.
 __global__ void ens_run_start(int* run_c) {
-  if (threadIdx.x == 0) __hip_atomic_store(run_c, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) *run_c = 0;
 }
 
 // Compensated FP64 mass/energy of the split state.
@@ -771,6 +852,12 @@ __global__ void __launch_bounds__(kDT) split_diag_final(const double* partials, 
   result[1] = E.s + (E.c + ce);
 }
 
+// Multiplier of the stage kernel's tile decomposition: for d >= 1 and
+// m = ceil(2^31 / d), hi32((2 t) m) = floor(t / d) whenever t d < 2^31 (the
+// error of m d over 2^31 is below d); the host keeps tile counts below 2^20
+// and d below 2^11.
+unsigned tile_magic(unsigned d) { return static_cast<unsigned>((0x80000000ull + d - 1) / d); }
+
 template <typename T, int TY, int OP>
 void launch_split_op(hipStream_t s, int kind, const SplitArgs<T>& a, const SplitStatics<T>& st,
                      const HaloScatter& hs) {
@@ -804,6 +891,8 @@ void launch_split(hipStream_t s, int kind, int op, SplitArgs<T> a, const SplitSt
                   const HaloScatter& hs, int tile_y) {
   a.nbx = (a.n + TX - 1) / TX;
   a.nby = (a.n + tile_y - 1) / tile_y;
+  a.mbx = tile_magic(static_cast<unsigned>(a.nbx));
+  a.mby = tile_magic(static_cast<unsigned>(a.nby));
   if (tile_y == 16) launch_split_ty<T, 16>(s, kind, op, a, st, hs);
   else if (tile_y == 32) launch_split_ty<T, 32>(s, kind, op, a, st, hs);
   else launch_split_ty<T, 8>(s, kind, op, a, st, hs);
@@ -946,6 +1035,16 @@ int split_create(const ccs_config& cfg, const HaloScatter& hs, hipStream_t st, S
     }
     s->tile_y = v;
   }
+  // the stage kernel's tile decomposition (tile_magic) is exact below 2^20
+  // tiles (n up to about 6700 with 8-row tiles)
+  {
+    const std::size_t nbx = (static_cast<std::size_t>(s->n) + 31) / 32, nby = (static_cast<std::size_t>(s->n) + 7) / 8;
+    if (nbx * nby * 6 >= (std::size_t{1} << 20)) {
+      *err = "split_create: too many tiles for the stage kernel; n is too large for the split path";
+      delete s;
+      return CCS_ERR_INVALID;
+    }
+  }
   // One allocation: the FP64 block (Pref, sqrtG, phis, diagnostics partials),
   // then 4 statics + 9 state arrays (padded) and 9 tendencies (interior).
   // Every array starts on a 256-byte boundary, so the size is the sum of the
@@ -1016,15 +1115,20 @@ int split_set_mesh(SplitState* s, hipStream_t st, double* const T[15], std::stri
     fs[d] = r * sg[d] * T[6][d];
   }
   std::vector<double> phis(T[5], T[5] + np);
+  // the WENO stage kernel stores its pointwise terms scaled by 1/8 (exact):
+  // its surface-geopotential static carries the same factor
+  std::vector<double> phis_k(phis);
+  if (s->cfg.spatial_op == CCS_OP_WENO)
+    for (double& x : phis_k) x *= 0.125;
   std::vector<char> stage;
   if (s->fp64) {
     upload_cast<double>(st, s->sg, sg, stage);
     upload_cast<double>(st, s->fs, fs, stage);
-    upload_cast<double>(st, s->phis, phis, stage);
+    upload_cast<double>(st, s->phis, phis_k, stage);
   } else {
     upload_cast<float>(st, s->sg, sg, stage);
     upload_cast<float>(st, s->fs, fs, stage);
-    upload_cast<float>(st, s->phis, phis, stage);
+    upload_cast<float>(st, s->phis, phis_k, stage);
   }
   std::vector<double> sq(T[0], T[0] + np);
   (void)hipMemcpyAsync(s->sqrtG_d, sq.data(), np * sizeof(double), hipMemcpyHostToDevice, st);
@@ -1368,7 +1472,9 @@ void ens_step_t(const EnsPlan* e, hipStream_t st, int scheme) {
     ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0}, run);
   }
   const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
-  hipLaunchKernelGGL(ens_rebase<T>, dim3(32, static_cast<unsigned>(e->nm)), dim3(256), 0, st, md, run,
+  // few blocks per member: the kernel runs every step and folds only every
+  // kRebaseSteps-th, so its cost is mostly the blocks that find nothing due
+  hipLaunchKernelGGL(ens_rebase<T>, dim3(8, static_cast<unsigned>(e->nm)), dim3(256), 0, st, md, run,
                      static_cast<const int*>(e->d_done));
   hipLaunchKernelGGL(ens_tick<T>, dim3(1), dim3(256), 0, st, md, e->nm, e->d_run, e->d_done);
 }
