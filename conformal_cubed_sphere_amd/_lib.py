@@ -115,6 +115,8 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "ccs_history_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D]),
     "ccs_history_close": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccs_diagnostics_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 11
+                                + [ctypes.c_char_p, ctypes.c_size_t]),
     "ccs_ensemble_create": (ctypes.c_int, [ctypes.POINTER(CcsEnsembleConfig), _I, _D, _D,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "ccs_ensemble_set_mesh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D]),

@@ -27,6 +27,8 @@
 #ifndef CCS_H
 #define CCS_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -273,6 +275,28 @@ int ccs_history_write(ccs_history* h, int time_slot, const double* phi,
                       const double* uc, const double* vc, const double* zonal,
                       const double* merid);
 int ccs_history_close(ccs_history* h);
+
+/* Total mass and total energy, as ccs_diagnostics computes them on the FP64
+ * parity path (calc_total_mass / calc_total_energy, diag_mod.F90:10-53, as
+ * compensated FP64 sums; the same kernels, so the same bits), of a padded
+ * state held in caller-owned device memory, without allocating anything.  No
+ * counterpart in the reference: this entry point is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  phiG, u, v and the static tables sqrtG, the G^-1
+ * entries ig11, ig12, ig21, ig22 and phis are device arrays of
+ * (n+2*halo)^2*6 doubles in the reference layout; result is a device array of
+ * 2 doubles (mass, energy) written on `stream` (a hipStream_t, NULL for the
+ * null stream); the call returns once the work is enqueued.  The partial sums
+ * live in a 16 KiB device array of the library, one per GPU, shared by every
+ * call on that GPU: calls that overlap in time on one GPU race.  Grids beyond
+ * 2^31 - 1 padded cells are rejected (CCS_ERR_INVALID); err, when not NULL,
+ * receives err_len bytes of text on failure. */
+int ccs_diagnostics_noalloc(int n, int halo, const double* phiG, const double* u,
+                            const double* v, const double* sqrtG,
+                            const double* ig11, const double* ig12,
+                            const double* ig21, const double* ig22,
+                            const double* phis, double* result, void* stream,
+                            char* err, size_t err_len);
 
 /* ------------------------------------------------------- batched ensemble */
 /* Many independent runs, each with its own resolution, mesh, initial state
