@@ -12,6 +12,7 @@
 // exactly as update_stat/correct_bdy_ghost define them; only the very first
 // stage after ccs_set_state reads the uploaded companions (initTestCase's).
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cmath>
@@ -75,6 +76,15 @@ const char* kVersion = "ccs-mi355x 0.1.0 (gfx950)";
 constexpr int kGraphSteps = 16;
 static_assert(kGraphSteps == kRebaseSteps, "a long graph holds exactly one rebase");
 
+// A roctx range over a scope: the host-side span of an entry point or a
+// stage enqueue, seen by rocprofv3 --marker-trace.
+struct Range {
+  explicit Range(const char* what) { roctxRangePush(what); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+  ~Range() { roctxRangePop(); }
+};
+
 // HIP events destroyed on every exit path.
 struct EventSet {
   std::vector<hipEvent_t> ev;
@@ -132,7 +142,9 @@ int enqueue_step_f64(ccs_handle* hd, bool expl, hipEvent_t* ev) {
   Prog<double>& B = hd->buf[1];
   Prog<double>& C = hd->buf[2];
   int stage = 0;
+  static const char* const kStageName[4] = {"ccs stage 1", "ccs stage 2", "ccs stage 3", "ccs stage 4"};
   auto launch = [&](int kind, bool e) {
+    const Range r(kStageName[stage]);
     if (ev) (void)hipEventRecord(ev[2 * stage], hd->stream);
     launch_stage_f64(hd->stream, kind, e, a, hd->st, hd->hs);
     if (ev) (void)hipEventRecord(ev[2 * stage + 1], hd->stream);
@@ -417,6 +429,7 @@ extern "C" int ccs_set_mesh_ext(ccs_handle* hd, const double* ig_ext) {
 extern "C" int ccs_set_state(ccs_handle* hd, const double* phiG, const double* u, const double* v,
                              const double* phi, const double* uc, const double* vc) {
   if (!hd) return CCS_ERR_INVALID;
+  const Range range("ccs_set_state");
   if (!phiG || !u || !v) return fail(hd, CCS_ERR_INVALID, "ccs_set_state: phiG, u, v are required");
   const bool expl = phi && uc && vc;
   if (!expl && (phi || uc || vc)) return fail(hd, CCS_ERR_INVALID, "ccs_set_state: give all of phi, uc, vc or none");
@@ -449,6 +462,7 @@ extern "C" int ccs_set_state(ccs_handle* hd, const double* phiG, const double* u
 
 extern "C" int ccs_step(ccs_handle* hd, int nsteps) {
   if (!hd) return CCS_ERR_INVALID;
+  const Range range("ccs_step");
   if (nsteps < 0) return fail(hd, CCS_ERR_INVALID, "ccs_step: nsteps < 0");
   if (!hd->mesh_set || !hd->state_set) return fail(hd, CCS_ERR_ORDER, "ccs_step before ccs_set_mesh/ccs_set_state");
   if (hd->cfg.spatial_op == CCS_OP_WENO_EXT && !hd->ext_set)
@@ -497,6 +511,7 @@ extern "C" int ccs_synchronize(ccs_handle* hd) {
 extern "C" int ccs_get_state(ccs_handle* hd, double* phiG, double* u, double* v, double* phi, double* uc,
                              double* vc, double* zonal, double* merid) {
   if (!hd) return CCS_ERR_INVALID;
+  const Range range("ccs_get_state");
   if (!hd->state_set) return fail(hd, CCS_ERR_ORDER, "ccs_get_state before ccs_set_state");
   HIPCHK(hd, hipSetDevice(hd->cfg.device));
   if (hd->cfg.precision != CCS_FP64_PARITY)
@@ -548,6 +563,7 @@ extern "C" int ccs_tendency(ccs_handle* hd, double* tp, double* tu, double* tv) 
 
 extern "C" int ccs_diagnostics(ccs_handle* hd, double* mass, double* energy) {
   if (!hd) return CCS_ERR_INVALID;
+  const Range range("ccs_diagnostics");
   if (!hd->mesh_set || !hd->state_set) return fail(hd, CCS_ERR_ORDER, "ccs_diagnostics before set_mesh/set_state");
   HIPCHK(hd, hipSetDevice(hd->cfg.device));
   double r[2] = {0.0, 0.0};
@@ -740,6 +756,7 @@ void tune_store(const std::string& key, const TuneChoice& c) {
 // all members still stepping, replayed from the one-step and kGraphSteps-step
 // graphs.
 int ens_run(ccs_ensemble* e, const int* target) {
+  const Range range("ccs_ensemble run");
   for (ccs_handle* h : e->m)
     if (!h->mesh_set || !h->state_set)
       return efail(e, CCS_ERR_ORDER, "ensemble run before every member's mesh and state are set");

@@ -47,6 +47,7 @@
 // OP = 1/2/3 instantiations: no phase 2, and phase 3 takes centred
 // differences of the phase-1 fields.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cmath>
@@ -959,10 +960,13 @@ void enqueue_step_t(SplitState* s, hipStream_t st, int scheme, hipEvent_t* ev) {
   SplitArgs<T> a = base_args<T>(s);
   const SplitStatics<T> S = statics_of<T>(s);
   int stage = 0;
+  static const char* const kStageName[4] = {"ccs stage 1", "ccs stage 2", "ccs stage 3", "ccs stage 4"};
   auto launch = [&](int kind) {
+    roctxRangePush(kStageName[stage]);
     if (ev) (void)hipEventRecord(ev[2 * stage], st);
     launch_split<T>(st, kind, s->cfg.spatial_op, a, S, s->hs, s->tile_y);
     if (ev) (void)hipEventRecord(ev[2 * stage + 1], st);
+    roctxRangePop();
     ++stage;
   };
   if (scheme == CCS_RK3_TVD) {

@@ -134,10 +134,13 @@ module ccs_gpu_mod
       real(c_double), intent(in) :: sqrtG(*), G(*), IG(*), A(*), IA(*), f(*), phis(*)
     end function
 
+    ! phi, uc, vc: c_loc of the companion arrays, or c_null_ptr for none
+    ! (the first step then derives them from phiG, u, v)
     integer(c_int) function ccs_set_state(h, phiG, u, v, phi, uc, vc) bind(C, name="ccs_set_state")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
-      real(c_double), intent(in) :: phiG(*), u(*), v(*), phi(*), uc(*), vc(*)
+      real(c_double), intent(in) :: phiG(*), u(*), v(*)
+      type(c_ptr), value :: phi, uc, vc
     end function
 
     integer(c_int) function ccs_step(h, nsteps) bind(C, name="ccs_step")

@@ -31,7 +31,17 @@
 !   history_file  history output of io_mod.F90:309-541 written at the
 !              initial state and every output (default 'ccsm_output.nc', the
 !              reference's name; '' for none).
-!   radius, gravity, omega, device, use_graph, final_state_file.
+!   radius, gravity, omega, device, use_graph.
+!   final_state_file    after the run, the state as raw stream records
+!              (n, halo, steps, dt; phiG, u, v, phi padded), a checkpoint.
+!   initial_state_file  start from such a checkpoint instead of the test
+!              case (same n and halo; the companions are derived from phiG,
+!              u, v as every step after the first does, so on the FP64 path
+!              a run resumed from a checkpoint continues bit for bit).
+!   run_record_file     after the run, one JSON line: grid, precision,
+!              scheme, steps, step-loop seconds, cell-updates/s, and the
+!              relative mass and energy change (MCR, ECR) of the final state;
+!              the same line is printed after the step-loop summary.
 module ccs_params_mod
   use iso_c_binding
   implicit none
@@ -55,6 +65,8 @@ module ccs_params_mod
   real(8) :: radius = 6371229.d0, gravity = 9.80616d0, omega = 7.292d-5
   integer :: device = 0, use_graph = 1
   character(len=200) :: final_state_file = ''
+  character(len=200) :: initial_state_file = ''
+  character(len=200) :: run_record_file = ''
   character(len=200) :: history_file = 'ccsm_output.nc'
   logical :: mesh_from_file = .false.
 
@@ -66,7 +78,7 @@ module ccs_params_mod
   namelist /domain/ mesh_file, xhalo, yhalo
   namelist /dynamic_opt/ reconstruct_scheme
   namelist /gpu_opt/ nx, precision, cfl, seed, n_bumps, mountain, u0, h_mean, radius, gravity, omega, device, &
-                     use_graph, final_state_file, history_file
+                     use_graph, final_state_file, history_file, initial_state_file, run_record_file
 
 contains
 
@@ -121,7 +133,11 @@ program ccsm
   real(8), allocatable, target :: lon_deg(:), lat_deg(:), xyz(:), jab(:)
   real(8), allocatable, target :: lon(:), lat(:), sinlon(:), coslon(:), sinlat(:), coslat(:)
   real(8), allocatable, target :: sqrtG(:), f(:), area(:), G(:), IG(:), A(:), IA(:)
-  real(8), allocatable :: phiG(:), u(:), v(:), phi(:), uc(:), vc(:), zonal(:), merid(:), phis(:)
+  real(8), allocatable :: phiG(:), u(:), v(:), zonal(:), merid(:), phis(:)
+  real(8), allocatable, target :: phi(:), uc(:), vc(:)
+  integer :: rn, rh, rsteps
+  real(8) :: rdt
+  character(len=1024) :: record
   real(8), allocatable, target :: jab_ext(:), ig_ext(:)
   type(ccs_metrics) :: m
   type(ccs_case_params) :: cp
@@ -249,7 +265,18 @@ program ccsm
     end if
     call ccs_check(ccs_set_mesh_ext(hnd, ig_ext), 'ccs_set_mesh_ext', hnd)
   end if
-  call ccs_check(ccs_set_state(hnd, phiG, u, v, phi, uc, vc), 'ccs_set_state', hnd)
+  if (len_trim(initial_state_file) > 0) then
+    ! resume from a checkpoint written by final_state_file
+    open(3, file=trim(initial_state_file), access='stream', form='unformatted', status='old')
+    read(3) rn, rh, rsteps, rdt
+    if (rn /= n .or. rh /= h) error stop 'initial_state_file: its n and halo differ from this run'
+    read(3) phiG, u, v, phi
+    close(3)
+    print *, 'State read from ', trim(initial_state_file), ' (written after ', rsteps, ' steps of dt ', rdt, ')'
+    call ccs_check(ccs_set_state(hnd, phiG, u, v, c_null_ptr, c_null_ptr, c_null_ptr), 'ccs_set_state', hnd)
+  else
+    call ccs_check(ccs_set_state(hnd, phiG, u, v, c_loc(phi), c_loc(uc), c_loc(vc)), 'ccs_set_state', hnd)
+  end if
 
   output_idx = 1
   total_output_num = int(nsteps * dt / history_interval)
@@ -305,6 +332,20 @@ program ccsm
   call system_clock(t_loop1)
   loop_s = dble(t_loop1 - t_loop0) / dble(rate)
   print *, 'Step loop: ', nsteps, ' steps in ', loop_s, ' s, cell-updates/s ', dble(ni) * nsteps / max(loop_s, 1d-9)
+  ! run record: one JSON line (printed, and written to run_record_file)
+  call ccs_check(ccs_diagnostics(hnd, mass, energy), 'ccs_diagnostics', hnd)
+  write(record, '(a,i0,a,i0,a,a,a,a,a,a,a,i0,a,es24.16e3,a,i0,a,es24.16e3,a,es24.16e3,a,es24.16e3,a,es24.16e3,a)') &
+    '{"n": ', n, ', "halo": ', h, ', "precision": "', trim(precision), '", "scheme": "', &
+    trim(adjustl(integral_scheme)), '", "operator": "', trim(adjustl(reconstruct_scheme)), '", "case": ', case_num, &
+    ', "dt": ', dt, ', "steps": ', nsteps, ', "loop_seconds": ', loop_s, ', "cell_updates_per_s": ', &
+    dble(ni) * nsteps / max(loop_s, 1d-9), ', "mcr": ', (mass - mass0) / mass0, ', "ecr": ', &
+    (energy - energy0) / energy0, '}'
+  print '(a)', trim(record)
+  if (len_trim(run_record_file) > 0) then
+    open(4, file=trim(run_record_file), status='replace', action='write')
+    write(4, '(a)') trim(record)
+    close(4)
+  end if
 
   if (len_trim(final_state_file) > 0) then
     call ccs_check(ccs_get_state(hnd, phiG, u, v, phi, uc, vc, zonal, merid), 'ccs_get_state', hnd)
