@@ -639,10 +639,13 @@ struct ccs_ensemble {
   EnsPlan* plan = nullptr;
   int tile_y = 0, persistent = 0, tuned_from_cache = 0;
   double stage_ms = 0.0;
-  // one step ([0]) and kGraphSteps steps ([1]) of a run: every launch of a
-  // step is the same for all steps (the run's step index lives on the device)
-  hipGraph_t graph[2]{};
-  hipGraphExec_t graph_exec[2]{};
+  // one step of a run, and kGraphSteps steps per rebase mask (the steps of
+  // the graph after which some member may rebase; see ens_enqueue_steps):
+  // every replay makes the same launches (the run's step index lives on the
+  // device)
+  hipGraph_t graph1 = nullptr;
+  hipGraphExec_t graph1_exec = nullptr;
+  std::map<unsigned, std::pair<hipGraph_t, hipGraphExec_t>> graphs16;
   std::vector<long long> done;  // per member: steps since its last ccs_ensemble_set_state
 };
 
@@ -661,12 +664,40 @@ int efail(ccs_ensemble* e, int code, const std::string& msg) {
   } while (0)
 
 void ens_drop_graphs(ccs_ensemble* e) {
-  for (int g = 0; g < 2; ++g) {
-    if (e->graph_exec[g]) (void)hipGraphExecDestroy(e->graph_exec[g]);
-    if (e->graph[g]) (void)hipGraphDestroy(e->graph[g]);
-    e->graph_exec[g] = nullptr;
-    e->graph[g] = nullptr;
+  if (e->graph1_exec) (void)hipGraphExecDestroy(e->graph1_exec);
+  if (e->graph1) (void)hipGraphDestroy(e->graph1);
+  e->graph1_exec = nullptr;
+  e->graph1 = nullptr;
+  for (auto& kv : e->graphs16) {
+    if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+    if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
   }
+  e->graphs16.clear();
+}
+
+// Capture `steps` steps with the given rebase mask into a graph.
+int ens_capture(ccs_ensemble* e, int steps, unsigned mask, hipGraph_t* g, hipGraphExec_t* x) {
+  ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  ens_enqueue_steps(e->plan, e->stream, e->cfg.scheme, steps, mask);
+  hipGraph_t gr = nullptr;
+  ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
+  *g = gr;
+  ENSCHK(e, hipGraphInstantiate(x, gr, nullptr, nullptr, 0));
+  return CCS_OK;
+}
+
+// Steps of the kGraphSteps steps from run step c0 after which some member
+// reaches a multiple of kRebaseSteps of its own steps: member m, at its
+// step count done[m] before the run, takes step c0 + k as its
+// (done[m] + c0 + k + 1)-th if c0 + k < target[m].
+unsigned ens_rebase_mask(const ccs_ensemble* e, const int* target, int c0) {
+  unsigned mask = 0;
+  for (std::size_t m = 0; m < e->m.size(); ++m) {
+    if (c0 >= target[m]) continue;
+    const int k = static_cast<int>((kRebaseSteps - 1 - (e->done[m] + c0) % kRebaseSteps) % kRebaseSteps);
+    if (c0 + k < target[m]) mask |= 1u << k;
+  }
+  return mask;
 }
 
 int ens_member(ccs_ensemble* e, int member) {
@@ -776,23 +807,33 @@ int ens_run(ccs_ensemble* e, const int* target) {
   if (rc != CCS_OK) return rc;
   const int scheme = e->cfg.scheme;
   int k = 0;
-  if (e->cfg.use_graph) {
-    for (int g = 0; g < 2; ++g) {
-      if (e->graph_exec[g]) continue;
-      ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      for (int s = 0; s < (g == 1 ? kGraphSteps : 1); ++s) ens_enqueue_step(e->plan, e->stream, scheme);
-      hipGraph_t gr = nullptr;
-      ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
-      e->graph[g] = gr;
-      ENSCHK(e, hipGraphInstantiate(&e->graph_exec[g], e->graph[g], nullptr, nullptr, 0));
+  for (; steps - k >= kGraphSteps; k += kGraphSteps) {
+    const unsigned mask = ens_rebase_mask(e, target, k);
+    if (!e->cfg.use_graph) {
+      ens_enqueue_steps(e->plan, e->stream, scheme, kGraphSteps, mask);
+      continue;
     }
-    while (k < steps) {
-      const int g = (steps - k >= kGraphSteps) ? 1 : 0;
-      ENSCHK(e, hipGraphLaunch(e->graph_exec[g], e->stream));
-      k += g ? kGraphSteps : 1;
+    auto it = e->graphs16.find(mask);
+    if (it == e->graphs16.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t x = nullptr;
+      rc = ens_capture(e, kGraphSteps, mask, &g, &x);
+      if (rc != CCS_OK) return rc;
+      it = e->graphs16.emplace(mask, std::make_pair(g, x)).first;
     }
+    ENSCHK(e, hipGraphLaunch(it->second.second, e->stream));
   }
-  for (; k < steps; ++k) ens_enqueue_step(e->plan, e->stream, scheme);
+  for (; k < steps; ++k) {
+    if (!e->cfg.use_graph) {
+      ens_enqueue_steps(e->plan, e->stream, scheme, 1, 0);
+      continue;
+    }
+    if (!e->graph1_exec) {
+      rc = ens_capture(e, 1, 0, &e->graph1, &e->graph1_exec);
+      if (rc != CCS_OK) return rc;
+    }
+    ENSCHK(e, hipGraphLaunch(e->graph1_exec, e->stream));
+  }
   ENSCHK(e, hipGetLastError());
   for (std::size_t m = 0; m < e->m.size(); ++m) e->done[m] += target[m];
   return CCS_OK;
@@ -924,20 +965,20 @@ extern "C" int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persiste
     e->tuned_from_cache = 1;
   } else {
     std::vector<int> tys, pers;
-    if (e->cfg.tile_y) tys.push_back(e->cfg.tile_y); else tys = {8, 16, 32};
+    if (e->cfg.tile_y) tys.push_back(e->cfg.tile_y); else tys = {16, 8, 32};
     if (e->cfg.persistent >= 0) pers.push_back(e->cfg.persistent); else pers = {0, 1};
     EventSet ev(2);
     ENSCHK(e, ev.create());
-    constexpr int kReps = 6;
+    constexpr int kReps = 10;
     for (int ty : tys) {
       for (int ps : pers) {
         EnsPlan* p = nullptr;
         rc = ens_plan_create(st.data(), nm, ty, ps, e->cfg.device, e->done.data(), &p, &e->err);
         if (rc != CCS_OK) break;
-        // the tendency launch has the cost of a stage and changes no state
-        ens_enqueue_tendency(p, e->stream);
+        // a stage-2 launch has the cost of a stage and changes no state
+        ens_enqueue_probe(p, e->stream);
         (void)hipEventRecord(ev[0], e->stream);
-        for (int r = 0; r < kReps; ++r) ens_enqueue_tendency(p, e->stream);
+        for (int r = 0; r < kReps; ++r) ens_enqueue_probe(p, e->stream);
         (void)hipEventRecord(ev[1], e->stream);
         float ms = 0.f;
         if (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
@@ -946,7 +987,9 @@ extern "C" int ccs_ensemble_autotune(ccs_ensemble* e, int* tile_y, int* persiste
           break;
         }
         const double mean = ms / kReps;
-        if (!best || mean < best_ms) {
+        // a plan must beat the best so far by 1 %: ties keep the earlier one
+        // (16-row tiles, one workgroup per tile come first)
+        if (!best || mean < 0.99 * best_ms) {
           if (best) ens_plan_destroy(best);
           best = p;
           best_ms = mean;

@@ -40,13 +40,17 @@ constexpr int kRebaseSteps = 16;
 // CUs x resident workgroups that loops over the tiles, else one workgroup
 // per tile).  Members keep their own buffers, statics and time step.
 //
-// A run advances member m by target[m] steps.  The launches of one step are
-// the same for every step of a run (so one captured graph serves them all):
-// a device counter holds the step index within the run, every stage kernel
-// skips the tiles of members that have taken their steps, the rebase kernel
-// folds the deviation of members whose own step count since the upload of
-// their state reaches a multiple of kRebaseSteps, and a one-workgroup kernel
-// advances the per-member step counts and the run's step index.
+// A run advances member m by target[m] steps, from graphs of 1 and of
+// kRebaseSteps steps whose launches are the same for every replay: a device
+// counter holds the run's step index at the start of the graph (advanced by
+// a one-workgroup kernel at the graph's end, with the per-member step
+// counts), every stage kernel skips the tiles of members that have taken
+// their steps, and the rebase kernel folds the deviation of members whose own
+// step count since the upload of their state reaches a multiple of
+// kRebaseSteps.  In a graph of kRebaseSteps steps the rebase kernel follows
+// only the steps named by a mask, which the host derives from the members'
+// step counts (one step of the graph when every member counts from the same
+// phase).
 struct EnsPlan;
 // steps_done[m]: the member's steps since the upload of its state (NULL: 0).
 int ens_plan_create(SplitState* const* mem, int nm, int tile_y, int persistent, int device,
@@ -60,8 +64,14 @@ int ens_plan_set_run(EnsPlan* e, hipStream_t st, const int* target, std::string*
 // The member's state was uploaded again: its step count restarts at zero
 // (drains st).
 int ens_plan_reset_member(EnsPlan* e, hipStream_t st, int member);
-void ens_enqueue_step(const EnsPlan* e, hipStream_t st, int scheme);
+// Enqueue `steps` (1 or kRebaseSteps) steps; rebase_mask: for a graph of
+// kRebaseSteps steps, bit k set where some member may reach its rebase count
+// with step k (a superset is correct); a single step always carries it.
+void ens_enqueue_steps(const EnsPlan* e, hipStream_t st, int scheme, int steps, unsigned rebase_mask);
 void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st);
+// One stage launch that leaves every member's state as it is (the stage-2
+// kernel, whose output buffer is dead between steps): the autotuner's probe.
+void ens_enqueue_probe(const EnsPlan* e, hipStream_t st);
 int ens_plan_grid(const EnsPlan* e);
 int ens_plan_tile_y(const EnsPlan* e);
 int ens_plan_persistent(const EnsPlan* e);

@@ -682,10 +682,11 @@ struct EnsStage {
   int inc_full; // RK4: 1 -> dt/a increment, 0 -> dt/(2a)
 };
 
-// Step index within the current run (device counter, advanced by ens_tick
-// in an earlier launch of the stream, so a plain uniform load sees it), or -1
-// where every member steps (the tendency launch of the autotuner).
-__device__ __forceinline__ int run_step(const int* __restrict__ run_c) { return run_c ? *run_c : -1; }
+// Step index within the current run: the index at the start of the graph
+// being replayed (device counter, advanced by ens_advance at the end of each
+// graph, so a plain uniform load sees it) plus the step's offset in the
+// graph; -1 where every member steps (the tendency launch of the autotuner).
+__device__ __forceinline__ int run_step(const int* __restrict__ base, int k) { return base ? *base + k : -1; }
 
 // One tile of the global list; false when its member has taken its steps
 // (members are ordered by decreasing step count, so every later tile of the
@@ -718,9 +719,9 @@ __device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, co
 template <typename T, int KIND, int TY, bool PERSIST>
 __global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restrict__ md,
                                                       const int* __restrict__ tile_member, int ntiles, EnsStage sel,
-                                                      const int* run_c) {
+                                                      const int* base, int koff) {
   const unsigned nb = gridDim.x, b = blockIdx.x;
-  const int step = run_step(run_c);
+  const int step = run_step(base, koff);
   if (!PERSIST) {
     (void)ens_tile<T, KIND, TY>(md, tile_member, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x), step);
     return;
@@ -754,14 +755,15 @@ __global__ void __launch_bounds__(256) split_rebase(double* __restrict__ pref64,
 
 // The same for the members of an ensemble (blockIdx.y = position in the
 // member table) whose own step count reaches a multiple of kRebaseSteps with
-// the step just taken: done[] holds each member's steps since its upload,
-// modulo kRebaseSteps (ens_tick advances it after this kernel).
+// step koff of the graph: done[] holds each member's steps since its upload,
+// modulo kRebaseSteps, as of the start of the graph (ens_advance updates it
+// at the graph's end).
 template <typename T>
-__global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict__ md, const int* run_c,
+__global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict__ md, const int* base, int koff,
                                                   const int* done) {
   const EnsMember<T>* M = md + blockIdx.y;
-  const int c = run_step(run_c);
-  if (c >= M->target || (done[M->member] + 1) % kRebaseSteps != 0) return;
+  const int c = run_step(base, koff);
+  if (c >= M->target || (done[M->member] + koff + 1) % kRebaseSteps != 0) return;
   T* dP = M->buf[0][0];
   for (unsigned long long d = blockIdx.x * 256ull + threadIdx.x; d < M->np; d += gridDim.x * 256ull) {
     const double p = M->pref64[d] + static_cast<double>(dP[d]);
@@ -771,21 +773,24 @@ __global__ void __launch_bounds__(256) ens_rebase(const EnsMember<T>* __restrict
   }
 }
 
-// End of a step: the step counters of the members that took it, then the
-// run's step index (one workgroup; every thread reads the index before
-// thread 0 advances it).
+// End of a graph of `steps` steps: every member's step count advances by the
+// steps it took in the graph, then the run's step index (one workgroup; every
+// thread reads the index before thread 0 advances it).
 template <typename T>
-__global__ void __launch_bounds__(256) ens_tick(const EnsMember<T>* __restrict__ md, int nm, int* run_c, int* done) {
-  const int c = *run_c;
-  for (int k = threadIdx.x; k < nm; k += 256)
-    if (c < md[k].target) done[md[k].member] = (done[md[k].member] + 1) % kRebaseSteps;
+__global__ void __launch_bounds__(256) ens_advance(const EnsMember<T>* __restrict__ md, int nm, int* base, int* done,
+                                                   int steps) {
+  const int c = *base;
+  for (int k = threadIdx.x; k < nm; k += 256) {
+    const int taken = min(max(md[k].target - c, 0), steps);
+    done[md[k].member] = (done[md[k].member] + taken) % kRebaseSteps;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) *run_c = c + 1;
+  if (threadIdx.x == 0) *base = c + steps;
 }
 
 // Start of a run: step index 0.
-__global__ void ens_run_start(int* run_c) {
-  if (threadIdx.x == 0) *run_c = 0;
+__global__ void ens_run_start(int* base) {
+  if (threadIdx.x == 0) *base = 0;
 }
 
 // Compensated FP64 mass/energy of the split state.
@@ -1320,7 +1325,7 @@ struct EnsPlan {
   void* dmem = nullptr;     // member table, tile -> member map, run step index, step counters
   void* d_desc = nullptr;
   int* d_tiles = nullptr;
-  int* d_run = nullptr;     // step index within the current run
+  int* d_run = nullptr;     // step index within the current run at the start of the graph being replayed
   int* d_done = nullptr;    // per member: steps since its upload, modulo kRebaseSteps
   std::vector<char> md_nat;           // member table in member order (EnsMember<T>)
   std::vector<int> tiles_per_member;
@@ -1330,31 +1335,31 @@ struct EnsPlan {
 namespace {
 
 template <typename T, int TY, bool PERSIST>
-void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run) {
+void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff) {
   const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
   const dim3 grid(static_cast<unsigned>(e->grid)), block(TX * TY);
   const int nt = e->ntiles;
   switch (kind) {
-    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
-    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
-    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
-    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
-    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
-    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run); break;
+    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
+    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
+    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
   }
 }
 
 template <typename T, int TY>
-void ens_launch_ty(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run) {
-  if (e->persistent) ens_launch_k<T, TY, true>(e, s, kind, sel, run);
-  else ens_launch_k<T, TY, false>(e, s, kind, sel, run);
+void ens_launch_ty(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff) {
+  if (e->persistent) ens_launch_k<T, TY, true>(e, s, kind, sel, run, koff);
+  else ens_launch_k<T, TY, false>(e, s, kind, sel, run, koff);
 }
 
 template <typename T>
-void ens_launch(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run) {
-  if (e->tile_y == 8) ens_launch_ty<T, 8>(e, s, kind, sel, run);
-  else if (e->tile_y == 32) ens_launch_ty<T, 32>(e, s, kind, sel, run);
-  else ens_launch_ty<T, 16>(e, s, kind, sel, run);
+void ens_launch(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff) {
+  if (e->tile_y == 8) ens_launch_ty<T, 8>(e, s, kind, sel, run, koff);
+  else if (e->tile_y == 32) ens_launch_ty<T, 32>(e, s, kind, sel, run, koff);
+  else ens_launch_ty<T, 16>(e, s, kind, sel, run, koff);
 }
 
 template <typename T, int TY>
@@ -1462,25 +1467,29 @@ int ens_set_run_t(EnsPlan* e, hipStream_t st, const int* target, std::string* er
   return CCS_OK;
 }
 
+// `steps` (1 or kRebaseSteps) steps: the stage launches of each, the rebase
+// after step k where bit k of rebase_mask is set (one step: always), then the
+// advance of the step counters.
 template <typename T>
-void ens_step_t(const EnsPlan* e, hipStream_t st, int scheme) {
+void ens_steps_t(const EnsPlan* e, hipStream_t st, int scheme, int steps, unsigned rebase_mask) {
   const int* run = e->d_run;
-  if (scheme == CCS_RK3_TVD) {
-    ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0}, run);
-    ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, run);
-    ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0}, run);
-  } else {
-    ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0}, run);
-    ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0}, run);
-    ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1}, run);
-    ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0}, run);
-  }
   const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
-  // few blocks per member: the kernel runs every step and folds only every
-  // kRebaseSteps-th, so its cost is mostly the blocks that find nothing due
-  hipLaunchKernelGGL(ens_rebase<T>, dim3(8, static_cast<unsigned>(e->nm)), dim3(256), 0, st, md, run,
-                     static_cast<const int*>(e->d_done));
-  hipLaunchKernelGGL(ens_tick<T>, dim3(1), dim3(256), 0, st, md, e->nm, e->d_run, e->d_done);
+  for (int k = 0; k < steps; ++k) {
+    if (scheme == CCS_RK3_TVD) {
+      ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0}, run, k);
+      ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, run, k);
+      ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0}, run, k);
+    } else {
+      ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0}, run, k);
+      ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0}, run, k);
+      ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1}, run, k);
+      ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0}, run, k);
+    }
+    if (steps == 1 || ((rebase_mask >> k) & 1u))
+      hipLaunchKernelGGL(ens_rebase<T>, dim3(8, static_cast<unsigned>(e->nm)), dim3(256), 0, st, md, run, k,
+                         static_cast<const int*>(e->d_done));
+  }
+  hipLaunchKernelGGL(ens_advance<T>, dim3(1), dim3(256), 0, st, md, e->nm, e->d_run, e->d_done, steps);
 }
 
 }  // namespace
@@ -1540,14 +1549,22 @@ int ens_plan_reset_member(EnsPlan* e, hipStream_t st, int member) {
   return CCS_OK;
 }
 
-void ens_enqueue_step(const EnsPlan* e, hipStream_t st, int scheme) {
-  if (e->fp64) ens_step_t<double>(e, st, scheme);
-  else ens_step_t<float>(e, st, scheme);
+void ens_enqueue_steps(const EnsPlan* e, hipStream_t st, int scheme, int steps, unsigned rebase_mask) {
+  if (e->fp64) ens_steps_t<double>(e, st, scheme, steps, rebase_mask);
+  else ens_steps_t<float>(e, st, scheme, steps, rebase_mask);
+}
+
+void ens_enqueue_probe(const EnsPlan* e, hipStream_t st) {
+  // SSP-RK3 stage 2: reads buffers 1 and 0, writes buffer 2, which holds
+  // only a stage-2 result between the stages of a step, so the state (buffer
+  // 0) is untouched
+  if (e->fp64) ens_launch<double>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, nullptr, 0);
+  else ens_launch<float>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, nullptr, 0);
 }
 
 void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st) {
-  if (e->fp64) ens_launch<double>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr);
-  else ens_launch<float>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr);
+  if (e->fp64) ens_launch<double>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr, 0);
+  else ens_launch<float>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr, 0);
 }
 
 int ens_plan_grid(const EnsPlan* e) { return e->grid; }

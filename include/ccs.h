@@ -351,9 +351,12 @@ int ccs_ensemble_step_members(ccs_ensemble* e, const int* nsteps);
 int ccs_ensemble_run_until(ccs_ensemble* e, double run_time);
 /* Steps member m has taken since its last ccs_ensemble_set_state. */
 int ccs_ensemble_steps_done(const ccs_ensemble* e, int member, long long* steps);
-/* Time tile heights 8/16/32 x {one workgroup per tile, resident grid} on the
- * members' tendency (no state change) and keep the fastest where the config
- * left the choice open; reports the choice and its mean stage time.  Runs by
+/* Time tile heights 16/8/32 x {one workgroup per tile, resident grid} on an
+ * SSP-RK3 stage-2 launch (it writes only the buffer that holds a stage-2
+ * result between the stages of a step, so no state changes) and keep the
+ * fastest where the config left the choice open (a plan must be 1 % faster
+ * than an earlier one to replace it); reports the choice and its mean stage
+ * time.  Runs by
  * itself on the first step call if not called.  The choice is cached per
  * device, precision, scheme, halo, multiset of member sizes and config
  * constraints for the life of the process, and in the file named by the
