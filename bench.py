@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-members", type=int, default=64)
-    ap.add_argument("--ens-steps", type=int, default=60)
+    ap.add_argument("--ens-steps", type=int, default=64)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--spatial-op", default="weno",
                     choices=["weno", "centred2", "centred4", "centred6", "weno_ext"],
@@ -354,7 +354,7 @@ def main():
     else:
         core.close()
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_bench(ccs, args.ens_members, args.ens_steps, 5)
+        out["ensemble"] = ensemble_bench(ccs, args.ens_members, args.ens_steps, 16)
     if not args.no_cpu:
         port = cpu_baseline(ccs, mesh, s0, dt, args.cpu_seconds, args.spatial_op)
         # the timed reference build holds the default operator module

@@ -643,8 +643,8 @@ struct ccs_ensemble {
   // the graph after which some member may rebase; see ens_enqueue_steps):
   // every replay makes the same launches (the run's step index lives on the
   // device)
-  hipGraph_t graph1 = nullptr;
-  hipGraphExec_t graph1_exec = nullptr;
+  hipGraph_t graph1[2]{};            // one step: without, with the rebase
+  hipGraphExec_t graph1_exec[2]{};
   std::map<unsigned, std::pair<hipGraph_t, hipGraphExec_t>> graphs16;
   std::vector<long long> done;  // per member: steps since its last ccs_ensemble_set_state
 };
@@ -664,10 +664,12 @@ int efail(ccs_ensemble* e, int code, const std::string& msg) {
   } while (0)
 
 void ens_drop_graphs(ccs_ensemble* e) {
-  if (e->graph1_exec) (void)hipGraphExecDestroy(e->graph1_exec);
-  if (e->graph1) (void)hipGraphDestroy(e->graph1);
-  e->graph1_exec = nullptr;
-  e->graph1 = nullptr;
+  for (int r = 0; r < 2; ++r) {
+    if (e->graph1_exec[r]) (void)hipGraphExecDestroy(e->graph1_exec[r]);
+    if (e->graph1[r]) (void)hipGraphDestroy(e->graph1[r]);
+    e->graph1_exec[r] = nullptr;
+    e->graph1[r] = nullptr;
+  }
   for (auto& kv : e->graphs16) {
     if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
     if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
@@ -824,15 +826,17 @@ int ens_run(ccs_ensemble* e, const int* target) {
     ENSCHK(e, hipGraphLaunch(it->second.second, e->stream));
   }
   for (; k < steps; ++k) {
+    // with the rebase only where some member's step count calls for it
+    const unsigned r = ens_rebase_mask(e, target, k) & 1u;
     if (!e->cfg.use_graph) {
-      ens_enqueue_steps(e->plan, e->stream, scheme, 1, 0);
+      ens_enqueue_steps(e->plan, e->stream, scheme, 1, r);
       continue;
     }
-    if (!e->graph1_exec) {
-      rc = ens_capture(e, 1, 0, &e->graph1, &e->graph1_exec);
+    if (!e->graph1_exec[r]) {
+      rc = ens_capture(e, 1, r, &e->graph1[r], &e->graph1_exec[r]);
       if (rc != CCS_OK) return rc;
     }
-    ENSCHK(e, hipGraphLaunch(e->graph1_exec, e->stream));
+    ENSCHK(e, hipGraphLaunch(e->graph1_exec[r], e->stream));
   }
   ENSCHK(e, hipGetLastError());
   for (std::size_t m = 0; m < e->m.size(); ++m) e->done[m] += target[m];

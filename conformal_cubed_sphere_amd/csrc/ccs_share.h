@@ -12,23 +12,31 @@
 
 namespace ccs {
 
-struct EnsShare {
-  unsigned first, end, stride;  // tiles first, first + stride, ... below end
+// Spacing of the claim counters in 32-bit words: one counter per 128-byte
+// line, so the groups' atomics do not share a cache line.
+constexpr unsigned kClaimStride = 32;
+// Counters per launch: one per group (at most 8 groups).
+constexpr unsigned kClaimGroups = 8;
+
+struct EnsGroup {
+  unsigned g;       // group of the workgroup: its claim counter
+  unsigned lo, hi;  // the group's contiguous tiles [lo, hi) of the global list
 };
 
-// Share of workgroup b of nb: the workgroups form min(nb, 8) groups (group
+// Group of workgroup b of nb: the workgroups form min(nb, 8) groups (group
 // g = b mod 8 runs on XCD g when nb >= 8, workgroups being dispatched round
-// robin over the 8 XCDs), group g owns the contiguous tiles [lo, hi) of the
-// global list and its workgroups take them in turn (first tile lo + b / groups,
-// stride the group's workgroup count).  Every tile belongs to exactly one
-// group and is visited by exactly one of its workgroups, for every nb >= 1.
-CCS_HD inline EnsShare ens_share(unsigned b, unsigned nb, unsigned ntiles) {
-  const unsigned ng = nb < 8u ? nb : 8u;
+// robin over the 8 XCDs), and group g owns the contiguous tiles [lo, hi).
+// Its workgroups claim them one at a time from the group's counter (tile
+// lo + the counter's old value, until that reaches hi), so every tile is
+// claimed exactly once and a slow workgroup takes fewer tiles.  The ranges
+// cover the list exactly, for every nb >= 1.
+CCS_HD inline EnsGroup ens_group(unsigned b, unsigned nb, unsigned ntiles) {
+  const unsigned ng = nb < kClaimGroups ? nb : kClaimGroups;
   const unsigned g = b % ng;
-  EnsShare s;
-  s.first = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * g) / ng) + b / ng;
-  s.end = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * (g + 1)) / ng);
-  s.stride = nb / ng + (g < nb % ng ? 1u : 0u);
+  EnsGroup s;
+  s.g = g;
+  s.lo = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * g) / ng);
+  s.hi = static_cast<unsigned>((static_cast<unsigned long long>(ntiles) * (g + 1)) / ng);
   return s;
 }
 

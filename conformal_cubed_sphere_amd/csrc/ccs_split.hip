@@ -667,7 +667,6 @@ struct EnsMember {
   T* buf[3][3];        // [buffer][dP, U, V]
   T* tend[9];
   T inc_half, inc_full;  // RK4 increments dt/(2a), dt/a
-  int tile0;             // first global tile of the member
   int target;            // steps the member takes in the current run
   int member;            // index of the member in the ensemble (its slot in the step counters)
   int hs[6][4][11];
@@ -685,18 +684,28 @@ struct EnsStage {
 // Step index within the current run: the index at the start of the graph
 // being replayed (device counter, advanced by ens_advance at the end of each
 // graph, so a plain uniform load sees it) plus the step's offset in the
-// graph; -1 where every member steps (the tendency launch of the autotuner).
-__device__ __forceinline__ int run_step(const int* __restrict__ base, int k) { return base ? *base + k : -1; }
+// graph.  The autotuner's launches, where every member steps, read a counter
+// that holds -1.
+__device__ __forceinline__ int run_step(const int* __restrict__ base, int k) { return *base + k; }
+
+// A tile of the global list: the member's slot in the member table (x) and
+// the panel, tile column and tile row packed as p | bx << 3 | by << 15 (y),
+// so a workgroup finds its tile with one load that depends on nothing but
+// its index.
+__host__ __device__ constexpr unsigned tile_pack(int p, int bx, int by) {
+  return static_cast<unsigned>(p) | (static_cast<unsigned>(bx) << 3) | (static_cast<unsigned>(by) << 15);
+}
 
 // One tile of the global list; false when its member has taken its steps
 // (members are ordered by decreasing step count, so every later tile of the
-// list is finished too).
+// list is finished too).  The member's fields are read before the step test,
+// so their loads share one round trip.
 template <typename T, int KIND, int TY>
-__device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, const int* __restrict__ tile_member,
+__device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, const uint2* __restrict__ tiles,
                                          const EnsStage& sel, const unsigned t, const int tid, const int step) {
-  const int m = tile_member[t];
-  const EnsMember<T>* M = md + m;
-  if (step >= M->target) return false;
+  const uint2 rec = tiles[t];
+  const EnsMember<T>* M = md + rec.x;
+  const int target = M->target;
   SplitArgs<T> a = M->a;
   a.in = Prog<const T>{M->buf[sel.in_b][0], M->buf[sel.in_b][1], M->buf[sel.in_b][2]};
   a.base = Prog<const T>{M->buf[sel.base_b][0], M->buf[sel.base_b][1], M->buf[sel.base_b][2]};
@@ -704,36 +713,57 @@ __device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, co
   for (int k = 0; k < 3; ++k) a.tend[k] = M->tend[sel.tend0 + k];
   for (int k = 0; k < 9; ++k) a.tacc[k] = M->tend[k];
   a.incs = sel.inc_full ? M->inc_full : M->inc_half;
-  const unsigned lt = t - static_cast<unsigned>(M->tile0);
-  const unsigned tq = lt / static_cast<unsigned>(a.nbx);
-  const int p = static_cast<int>(tq / static_cast<unsigned>(a.nby));
-  const int i0 = static_cast<int>(lt - tq * a.nbx) * TX;
-  const int j0 = static_cast<int>(tq - static_cast<unsigned>(p) * a.nby) * TY;
-  stage_tile<T, KIND, TY, 0>(a, M->st, M->hs[p], p, i0, j0, tid);
+  const SplitStatics<T> st = M->st;
+  // the loads phase 1 depends on are issued with the step test's, in one
+  // round trip (the compiler would otherwise defer them past the test)
+  asm volatile("" ::"s"(target), "s"(a.n), "s"(a.h), "s"(a.in.phiG), "s"(a.in.u), "s"(a.in.v), "s"(st.pref),
+               "s"(st.sg), "s"(st.phis), "s"(rec.y));
+  if (step >= target) return false;
+  const int p = static_cast<int>(rec.y & 7u);
+  const int i0 = static_cast<int>((rec.y >> 3) & 0xFFFu) * TX;
+  const int j0 = static_cast<int>(rec.y >> 15) * TY;
+  stage_tile<T, KIND, TY, 0>(a, st, M->hs[p], p, i0, j0, tid);
   return true;
 }
 
 // PERSIST = false: one workgroup per tile, in the XCD-aware order.
-// PERSIST = true: a grid of resident workgroups looping over their share of
-// the global tile list (ens_share).
+// PERSIST = true: a grid of resident workgroups; each belongs to a group
+// (ens_group, one per XCD) owning a contiguous run of the global tile list,
+// and claims the group's tiles one at a time from the group's counter in
+// `claim`, so the tiles spread over the workgroups as they finish.  Workgroup
+// 0 clears the counters of the next launch on the stream (`claim_next`): its
+// kernel starts after this one ends, and the launch that used them before
+// has ended.
 template <typename T, int KIND, int TY, bool PERSIST>
-__global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restrict__ md,
-                                                      const int* __restrict__ tile_member, int ntiles, EnsStage sel,
-                                                      const int* base, int koff) {
+__global__ void __launch_bounds__(TX * TY, (PERSIST && sizeof(T) == 4 && TY >= 16) ? 8 : 1) stage_ens(const EnsMember<T>* __restrict__ md,
+                                                      const uint2* __restrict__ tiles, int ntiles, EnsStage sel,
+                                                      const int* __restrict__ base, int koff,
+                                                      unsigned* __restrict__ claim, unsigned* __restrict__ claim_next) {
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const int step = run_step(base, koff);
   if (!PERSIST) {
-    (void)ens_tile<T, KIND, TY>(md, tile_member, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x), step);
+    (void)ens_tile<T, KIND, TY>(md, tiles, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x), step);
     return;
   }
-  const EnsShare sh = ens_share(b, nb, static_cast<unsigned>(ntiles));
-  for (unsigned t = sh.first; t < sh.end; t += sh.stride) {
-    __syncthreads();  // the previous tile's LDS reads are done
+  if (b == 0 && threadIdx.x < kClaimGroups) claim_next[threadIdx.x * kClaimStride] = 0u;
+  const EnsGroup gr = ens_group(b, nb, static_cast<unsigned>(ntiles));
+  // the claimed tile, through LDS; the claim for tile k + 1 is issued at the
+  // start of tile k, its result stored at the end, so its latency is hidden
+  __shared__ unsigned s_tile[2];
+  if (threadIdx.x == 0) s_tile[0] = gr.lo + atomicAdd(claim + gr.g * kClaimStride, 1u);
+  for (unsigned k = 0;; ++k) {
+    __syncthreads();  // s_tile[k & 1] is stored; the previous tile's LDS reads are done
+    // uniform: the tile's loads go through the scalar unit
+    const unsigned t = __builtin_amdgcn_readfirstlane(s_tile[k & 1u]);
+    if (t >= gr.hi) break;
+    unsigned nxt = 0;
+    if (threadIdx.x == 0) nxt = gr.lo + atomicAdd(claim + gr.g * kClaimStride, 1u);
     // keep the thread-index arithmetic inside the loop: hoisting it out would
     // hold ~30 more VGPRs for the whole kernel and halve the occupancy
     int tid = static_cast<int>(threadIdx.x);
     asm volatile("" : "+v"(tid));
-    if (!ens_tile<T, KIND, TY>(md, tile_member, sel, t, tid, step)) break;
+    if (!ens_tile<T, KIND, TY>(md, tiles, sel, t, tid, step)) break;
+    if (threadIdx.x == 0) s_tile[(k + 1u) & 1u] = nxt;
   }
 }
 
@@ -1322,44 +1352,84 @@ int split_diagnostics(SplitState* s, hipStream_t st, bool, double* result, std::
 struct EnsPlan {
   bool fp64 = false;
   int tile_y = 16, persistent = 0, grid = 0, ntiles = 0, nm = 0;
-  void* dmem = nullptr;     // member table, tile -> member map, run step index, step counters
+  int rebase_gx = 8;        // rebase grid: workgroups per member
+  void* dmem = nullptr;     // member table, tile records, run step index, probe index, step counters
   void* d_desc = nullptr;
-  int* d_tiles = nullptr;
+  uint2* d_tiles = nullptr; // per tile: member slot, packed panel / tile column / tile row (tile_pack)
   int* d_run = nullptr;     // step index within the current run at the start of the graph being replayed
+  int* d_probe = nullptr;   // -1: the step index of the autotuner's launches (every member steps)
+  // persistent grid: the claim counters of kClaimSlots launches, used in
+  // turn: slot s by the s-th stage of a step (its kernel clears slot s + 1
+  // modulo the stages of the step), kProbeSlot and kProbeSlot + 1 by the
+  // autotuner's launches in turn
+  unsigned* d_claim = nullptr;
+  mutable int probe_turn = 0;
   int* d_done = nullptr;    // per member: steps since its upload, modulo kRebaseSteps
   std::vector<char> md_nat;           // member table in member order (EnsMember<T>)
-  std::vector<int> tiles_per_member;
+  std::vector<int> tiles_per_member, nbx, nby;
   std::vector<int> target;            // targets of the table on the device (empty: none uploaded)
 };
 
 namespace {
 
+// The tile records of a member in table slot `slot`: x fastest, then y,
+// then panel (the order of its cells in the padded layout).
+void append_tiles(std::vector<uint2>& tiles, int slot, int nbx, int nby) {
+  for (int p = 0; p < 6; ++p)
+    for (int by = 0; by < nby; ++by)
+      for (int bx = 0; bx < nbx; ++bx) tiles.push_back(uint2{static_cast<unsigned>(slot), tile_pack(p, bx, by)});
+}
+
+constexpr int kClaimSlots = 6;
+constexpr int kProbeSlot = 4;
+
+// claim counters of slot `slot` and of the slot its launch clears
+struct Claims {
+  unsigned *cur, *next;
+};
+Claims claims(const EnsPlan* e, int slot, int next) {
+  return Claims{e->d_claim + slot * kClaimGroups * kClaimStride, e->d_claim + next * kClaimGroups * kClaimStride};
+}
+
 template <typename T, int TY, bool PERSIST>
-void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff) {
+void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff,
+                  Claims c) {
   const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
   const dim3 grid(static_cast<unsigned>(e->grid)), block(TX * TY);
   const int nt = e->ntiles;
   switch (kind) {
-    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
-    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
-    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
-    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
-    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
-    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff); break;
+    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
+    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
+    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
   }
 }
 
 template <typename T, int TY>
-void ens_launch_ty(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff) {
-  if (e->persistent) ens_launch_k<T, TY, true>(e, s, kind, sel, run, koff);
-  else ens_launch_k<T, TY, false>(e, s, kind, sel, run, koff);
+void ens_launch_ty(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff,
+                   Claims c) {
+  if (e->persistent) ens_launch_k<T, TY, true>(e, s, kind, sel, run, koff, c);
+  else ens_launch_k<T, TY, false>(e, s, kind, sel, run, koff, c);
 }
 
+// stage `stage` of `nstages` of a step (claim slot `stage`, clearing the
+// next stage's), or an autotuner launch (stage < 0: the probe slots in turn)
 template <typename T>
-void ens_launch(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff) {
-  if (e->tile_y == 8) ens_launch_ty<T, 8>(e, s, kind, sel, run, koff);
-  else if (e->tile_y == 32) ens_launch_ty<T, 32>(e, s, kind, sel, run, koff);
-  else ens_launch_ty<T, 16>(e, s, kind, sel, run, koff);
+void ens_launch(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel, const int* run, int koff, int stage,
+                int nstages) {
+  Claims c;
+  if (stage >= 0) {
+    c = claims(e, stage, (stage + 1) % nstages);
+  } else {
+    const int k = e->probe_turn;
+    e->probe_turn ^= 1;
+    c = claims(e, kProbeSlot + k, kProbeSlot + (k ^ 1));
+  }
+  if (e->tile_y == 8) ens_launch_ty<T, 8>(e, s, kind, sel, run, koff, c);
+  else if (e->tile_y == 32) ens_launch_ty<T, 32>(e, s, kind, sel, run, koff, c);
+  else ens_launch_ty<T, 16>(e, s, kind, sel, run, koff, c);
 }
 
 template <typename T, int TY>
@@ -1374,8 +1444,11 @@ template <typename T>
 int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long long* done0, std::string* err) {
   const int TY = e->tile_y;
   std::vector<EnsMember<T>> md(nm);
-  std::vector<int> tiles;
+  std::vector<uint2> tiles;
   e->tiles_per_member.resize(nm);
+  e->nbx.resize(nm);
+  e->nby.resize(nm);
+  unsigned long long np_max = 0;
   for (int m = 0; m < nm; ++m) {
     const SplitState* s = mem[m];
     EnsMember<T>& d = md[m];
@@ -1391,34 +1464,46 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long
     d.np = s->np;
     d.inc_half = static_cast<T>(0.5 * s->cfg.dt / s->cfg.radius);
     d.inc_full = static_cast<T>(s->cfg.dt / s->cfg.radius);
-    d.tile0 = static_cast<int>(tiles.size());
     d.target = 0;
     d.member = m;
     for (int q = 0; q < 6; ++q)
       for (int r = 0; r < 4; ++r)
         for (int k = 0; k < 11; ++k) d.hs[q][r][k] = s->hs.r[q][r][k];
     e->tiles_per_member[m] = 6 * d.a.nbx * d.a.nby;
-    tiles.insert(tiles.end(), static_cast<std::size_t>(e->tiles_per_member[m]), m);
+    e->nbx[m] = d.a.nbx;
+    e->nby[m] = d.a.nby;
+    np_max = std::max(np_max, d.np);
+    append_tiles(tiles, m, d.a.nbx, d.a.nby);
   }
   e->ntiles = static_cast<int>(tiles.size());
+  // the rebase: about four cells per thread of the largest member
+  e->rebase_gx = static_cast<int>(std::min<unsigned long long>(2048, std::max<unsigned long long>(8, (np_max + 1023) / 1024)));
   auto rup = [](std::size_t b) { return (b + 255) / 256 * 256; };
-  const std::size_t bd = rup(md.size() * sizeof(EnsMember<T>)), bt = rup(tiles.size() * sizeof(int));
-  const std::size_t bytes = bd + bt + rup(sizeof(int)) + rup(nm * sizeof(int));
+  const std::size_t bd = rup(md.size() * sizeof(EnsMember<T>)), bt = rup(tiles.size() * sizeof(uint2));
+  const std::size_t bc = kClaimSlots * kClaimGroups * kClaimStride * sizeof(unsigned);
+  const std::size_t bytes = bd + bt + 2 * rup(sizeof(int)) + rup(nm * sizeof(int)) + bc;
   if (hipMalloc(&e->dmem, bytes) != hipSuccess) {
     *err = "ensemble: device allocation failed";
     return CCS_ERR_NOMEM;
   }
   char* p = static_cast<char*>(e->dmem);
   e->d_desc = p;
-  e->d_tiles = reinterpret_cast<int*>(p + bd);
+  e->d_tiles = reinterpret_cast<uint2*>(p + bd);
   e->d_run = reinterpret_cast<int*>(p + bd + bt);
-  e->d_done = reinterpret_cast<int*>(p + bd + bt + rup(sizeof(int)));
+  e->d_probe = reinterpret_cast<int*>(p + bd + bt + rup(sizeof(int)));
+  e->d_done = reinterpret_cast<int*>(p + bd + bt + 2 * rup(sizeof(int)));
+  e->d_claim = reinterpret_cast<unsigned*>(p + bd + bt + 2 * rup(sizeof(int)) + rup(nm * sizeof(int)));
+  const std::vector<unsigned> claim0(bc / sizeof(unsigned), 0u);
   e->md_nat.assign(reinterpret_cast<const char*>(md.data()), reinterpret_cast<const char*>(md.data() + nm));
   std::vector<int> done(nm, 0);
   for (int m = 0; m < nm && done0; ++m) done[m] = static_cast<int>(done0[m] % kRebaseSteps);
+  const int minus_one = -1, zero = 0;
   if (hipMemcpy(e->d_desc, md.data(), md.size() * sizeof(EnsMember<T>), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->d_done, done.data(), nm * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_run, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_probe, &minus_one, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_done, done.data(), nm * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_claim, claim0.data(), bc, hipMemcpyHostToDevice) != hipSuccess) {
     *err = "ensemble: table upload failed";
     return CCS_ERR_DEVICE;
   }
@@ -1448,18 +1533,17 @@ int ens_set_run_t(EnsPlan* e, hipStream_t st, const int* target, std::string* er
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return target[x] > target[y]; });
   const auto* nat = reinterpret_cast<const EnsMember<T>*>(e->md_nat.data());
   std::vector<EnsMember<T>> md(nm);
-  std::vector<int> tiles;
+  std::vector<uint2> tiles;
   tiles.reserve(static_cast<std::size_t>(e->ntiles));
   for (int k = 0; k < nm; ++k) {
     const int m = order[k];
     md[k] = nat[m];
     md[k].target = target[m];
-    md[k].tile0 = static_cast<int>(tiles.size());
-    tiles.insert(tiles.end(), static_cast<std::size_t>(e->tiles_per_member[m]), k);
+    append_tiles(tiles, k, e->nbx[m], e->nby[m]);
   }
   if (hipStreamSynchronize(st) != hipSuccess ||
       hipMemcpy(e->d_desc, md.data(), nm * sizeof(EnsMember<T>), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) {
     *err = "ensemble: run table upload failed";
     return CCS_ERR_DEVICE;
   }
@@ -1468,26 +1552,26 @@ int ens_set_run_t(EnsPlan* e, hipStream_t st, const int* target, std::string* er
 }
 
 // `steps` (1 or kRebaseSteps) steps: the stage launches of each, the rebase
-// after step k where bit k of rebase_mask is set (one step: always), then the
-// advance of the step counters.
+// after step k where bit k of rebase_mask is set, then the advance of the
+// step counters.
 template <typename T>
 void ens_steps_t(const EnsPlan* e, hipStream_t st, int scheme, int steps, unsigned rebase_mask) {
   const int* run = e->d_run;
   const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
   for (int k = 0; k < steps; ++k) {
     if (scheme == CCS_RK3_TVD) {
-      ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0}, run, k);
-      ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, run, k);
-      ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0}, run, k);
+      ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0}, run, k, 0, 3);
+      ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, run, k, 1, 3);
+      ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0}, run, k, 2, 3);
     } else {
-      ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0}, run, k);
-      ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0}, run, k);
-      ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1}, run, k);
-      ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0}, run, k);
+      ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0}, run, k, 0, 4);
+      ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0}, run, k, 1, 4);
+      ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1}, run, k, 2, 4);
+      ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0}, run, k, 3, 4);
     }
-    if (steps == 1 || ((rebase_mask >> k) & 1u))
-      hipLaunchKernelGGL(ens_rebase<T>, dim3(8, static_cast<unsigned>(e->nm)), dim3(256), 0, st, md, run, k,
-                         static_cast<const int*>(e->d_done));
+    if ((rebase_mask >> k) & 1u)
+      hipLaunchKernelGGL(ens_rebase<T>, dim3(static_cast<unsigned>(e->rebase_gx), static_cast<unsigned>(e->nm)),
+                         dim3(256), 0, st, md, run, k, static_cast<const int*>(e->d_done));
   }
   hipLaunchKernelGGL(ens_advance<T>, dim3(1), dim3(256), 0, st, md, e->nm, e->d_run, e->d_done, steps);
 }
@@ -1558,13 +1642,13 @@ void ens_enqueue_probe(const EnsPlan* e, hipStream_t st) {
   // SSP-RK3 stage 2: reads buffers 1 and 0, writes buffer 2, which holds
   // only a stage-2 result between the stages of a step, so the state (buffer
   // 0) is untouched
-  if (e->fp64) ens_launch<double>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, nullptr, 0);
-  else ens_launch<float>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, nullptr, 0);
+  if (e->fp64) ens_launch<double>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, e->d_probe, 0, -1, 0);
+  else ens_launch<float>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, e->d_probe, 0, -1, 0);
 }
 
 void ens_enqueue_tendency(const EnsPlan* e, hipStream_t st) {
-  if (e->fp64) ens_launch<double>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr, 0);
-  else ens_launch<float>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, nullptr, 0);
+  if (e->fp64) ens_launch<double>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, e->d_probe, 0, -1, 0);
+  else ens_launch<float>(e, st, kTendOnly, EnsStage{0, 0, 0, 0, 0}, e->d_probe, 0, -1, 0);
 }
 
 int ens_plan_grid(const EnsPlan* e) { return e->grid; }

@@ -41,7 +41,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64_fast"])
     ap.add_argument("--scheme", default="RK3_TVD", choices=["RK3_TVD", "RK4"])
     ap.add_argument("--tile-y", type=int, default=0, help="8, 16, 32 or 0 (autotune)")

@@ -1,8 +1,9 @@
-"""The persistent ensemble kernel splits the global tile list over its
-workgroups with ccs::ens_share (csrc/ccs_share.h).  Compile that header on
-the CPU and check that every tile is visited by exactly one workgroup for
+"""The persistent ensemble kernel splits the global tile list into group
+ranges with ccs::ens_group (csrc/ccs_share.h), whose workgroups claim tiles
+from a counter per group.  Compile that header on the CPU and check, for
 every grid size 1..80 (grids smaller than the 8 XCD groups included) and
-tile counts 1..300."""
+tile counts 1..300, that the ranges cover every tile once and that a
+simulated run of claims processes every tile exactly once."""
 import os
 import subprocess
 
