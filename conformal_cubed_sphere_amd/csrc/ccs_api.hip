@@ -49,6 +49,7 @@ struct ccs_handle {
   double* partials = nullptr;
   double* result = nullptr;
   double* face_speed = nullptr;  // CCS_OP_WENO_EXT: x-face then y-face G^-1 entries
+  int f64_tile_y = 8;            // rows per tile of the FP64 stage kernel (stage_f64_tile_y)
   // split-precision path
   SplitState* split = nullptr;
   // Step graphs.  Every step leaves the state in the same buffer, so a graph
@@ -146,7 +147,7 @@ int enqueue_step_f64(ccs_handle* hd, bool expl, hipEvent_t* ev) {
   auto launch = [&](int kind, bool e) {
     const Range r(kStageName[stage]);
     if (ev) (void)hipEventRecord(ev[2 * stage], hd->stream);
-    launch_stage_f64(hd->stream, kind, e, a, hd->st, hd->hs);
+    launch_stage_f64(hd->stream, kind, e, a, hd->st, hd->hs, hd->f64_tile_y);
     if (ev) (void)hipEventRecord(ev[2 * stage + 1], hd->stream);
     ++stage;
   };
@@ -284,6 +285,12 @@ extern "C" int ccs_create(const ccs_config* cfg, ccs_handle** out) {
   }
   int rc = CCS_OK;
   if (cfg->precision == CCS_FP64_PARITY) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device) != hipSuccess || cus <= 0) {
+      ccs_destroy(hd);
+      return CCS_ERR_DEVICE;
+    }
+    hd->f64_tile_y = stage_f64_tile_y(hd->n, cus);
     // statics 15 + state 9 + explicit 3 + derived 5 (padded), tendencies 9
     // (interior), diagnostics partials and result; the carving below walks
     // the same list and must end exactly at the end of the allocation
@@ -551,7 +558,7 @@ extern "C" int ccs_tendency(ccs_handle* hd, double* tp, double* tu, double* tv) 
   a.in = Prog<const double>{hd->buf[0].phiG, hd->buf[0].u, hd->buf[0].v};
   a.base = a.in;
   a.tend[0] = hd->tend[0]; a.tend[1] = hd->tend[1]; a.tend[2] = hd->tend[2];
-  launch_stage_f64(hd->stream, kTendOnly, hd->explicit_valid, a, hd->st, hd->hs);
+  launch_stage_f64(hd->stream, kTendOnly, hd->explicit_valid, a, hd->st, hd->hs, hd->f64_tile_y);
   HIPCHK(hd, hipGetLastError());
   const std::size_t bytes = hd->ni * sizeof(double);
   if (tp) HIPCHK(hd, hipMemcpyAsync(tp, hd->tend[0], bytes, hipMemcpyDeviceToHost, hd->stream));

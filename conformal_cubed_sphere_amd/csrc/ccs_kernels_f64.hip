@@ -12,7 +12,10 @@
 // correctly rounded ones), which keeps the branchy WENO-Z weight selection
 // (spatial_operators_mod.F90:348-355) on the same side as the CPU reference.
 //
-// Tiling: a 256-thread workgroup owns a TX x TY tile of one panel.  Phase 1
+// Tiling: a 256-thread workgroup owns a TX x TY tile of one panel (TY = 8,
+// or 7 where a grid of such tiles fits one round of the GPU's workgroup
+// slots: 487 faces then take two passes of the workgroup, not three, which
+// shortens a launch that is one round of workgroups long).  Phase 1
 // evaluates the pointwise terms (phi, contravariant winds, Bernoulli function,
 // mass fluxes) on the tile plus a 3-cell cross-shaped border into LDS.
 // Phase 2 evaluates every x- and y-face of the tile once (six WENO-Z
@@ -27,11 +30,8 @@ namespace ccs {
 namespace {
 
 constexpr int TX = 32;
-constexpr int TY = 8;
-constexpr int NT = TX * TY;
+constexpr int NT = 256;
 constexpr int EX = TX + 6;
-constexpr int EY = TY + 6;
-constexpr int NE = EX * EY;
 
 __device__ __forceinline__ long gidx(int i, int j, int p, int h, int P) {
   return static_cast<long>(i + h) + static_cast<long>(P) * (static_cast<long>(j + h) + static_cast<long>(P) * p);
@@ -96,9 +96,12 @@ __device__ __forceinline__ double d6(double m3, double m2, double m1, double p1,
 // Spatial operator variants (enum ccs_spatial_op): 0 WENO-Z, 1/2/3 centred
 // 2nd/4th/6th order, 4 WENO-Z with the extended-lattice metric and upwind
 // face winds in the vorticity.
-template <int KIND, bool EXPL, int OP>
+template <int KIND, bool EXPL, int OP, int TY>
 __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, HaloScatter hs) {
   constexpr bool kWeno = (OP == 0 || OP == 4);
+  constexpr int EY = TY + 6;
+  constexpr int NE = EX * EY;
+  static_assert(TX * TY <= NT, "one thread per tile cell");
   // LDS, sized so that four workgroups fit a CU.  Fields read along both
   // directions cover the extended tile (row-major [lj][li], EX x EY); fields
   // read only along x cover the tile's rows (XB: [lj-3][li], EX x TY), those
@@ -278,7 +281,7 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
   // ---- phase 3: cells
   const int tx = tid % TX, ty = tid / TX;
   const int i = i0 + tx, j = j0 + ty;
-  if (i >= n || j >= n) return;
+  if (ty >= TY || i >= n || j >= n) return;
   const double dx = a.dx;
   const double rdx = a.rdx, r60 = a.r60dx;
   const int lxc = ty * EX + (tx + 3);   // x band, this cell
@@ -499,8 +502,13 @@ __global__ void __launch_bounds__(kDiagThreads) diag_final_f64(const double* par
 
 }  // namespace
 
+int stage_f64_tile_y(int n, int cus) {
+  const long wg7 = static_cast<long>((n + TX - 1) / TX) * ((n + 6) / 7) * 6;
+  return wg7 <= 4L * cus ? 7 : 8;  // four workgroups fit a CU
+}
+
 void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_in, const StaticsF64& st,
-                      const HaloScatter& hs) {
+                      const HaloScatter& hs, int tile_y) {
   StageArgsF64 a = a_in;
   a.rdx = 1.0 / a.dx;            // correctly rounded reciprocals for div_known
   a.r60dx = 1.0 / (60.0 * a.dx);
@@ -508,12 +516,18 @@ void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_
   a.rdx2 = 1.0 / a.dx2;
   a.dx12 = 12.0 * a.dx;
   a.rdx12 = 1.0 / a.dx12;
-  const dim3 grid(static_cast<unsigned>(((a.n + TX - 1) / TX) * ((a.n + TY - 1) / TY) * 6));
+  const int ty = tile_y == 7 ? 7 : 8;
+  const dim3 grid(static_cast<unsigned>(((a.n + TX - 1) / TX) * ((a.n + ty - 1) / ty) * 6));
   const dim3 block(NT);
-#define CCS_LAUNCH_OP(K, O)                                                                \
-  do {                                                                                    \
-    if (expl) hipLaunchKernelGGL((stage_f64<K, true, O>), grid, block, 0, s, a, st, hs);  \
-    else hipLaunchKernelGGL((stage_f64<K, false, O>), grid, block, 0, s, a, st, hs);      \
+#define CCS_LAUNCH_TY(K, E, O)                                                           \
+  do {                                                                                  \
+    if (ty == 7) hipLaunchKernelGGL((stage_f64<K, E, O, 7>), grid, block, 0, s, a, st, hs); \
+    else hipLaunchKernelGGL((stage_f64<K, E, O, 8>), grid, block, 0, s, a, st, hs);         \
+  } while (0)
+#define CCS_LAUNCH_OP(K, O)                       \
+  do {                                           \
+    if (expl) CCS_LAUNCH_TY(K, true, O);         \
+    else CCS_LAUNCH_TY(K, false, O);             \
   } while (0)
 #define CCS_LAUNCH(K)                      \
   do {                                     \
@@ -535,6 +549,7 @@ void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_
   }
 #undef CCS_LAUNCH
 #undef CCS_LAUNCH_OP
+#undef CCS_LAUNCH_TY
 }
 
 void launch_derived_f64(hipStream_t s, const Prog<const double>& q, const StaticsF64& st, int n, int h,

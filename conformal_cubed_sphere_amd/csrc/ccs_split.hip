@@ -51,6 +51,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <vector>
 
@@ -688,40 +689,63 @@ struct EnsStage {
 // that holds -1.
 __device__ __forceinline__ int run_step(const int* __restrict__ base, int k) { return *base + k; }
 
-// A tile of the global list: the member's slot in the member table (x) and
-// the panel, tile column and tile row packed as p | bx << 3 | by << 15 (y),
-// so a workgroup finds its tile with one load that depends on nothing but
-// its index.
+// A tile of the global list, with what its first phase needs, so that a
+// workgroup issues its tile's loads after one load that depends on nothing
+// but its index.  A member's 13 padded arrays Pref, sqrtg, phis, a sqrtg f
+// and the three state buffers (dP, U, V) follow each other at a fixed
+// stride; the first lies `fbase` bytes from the member table (an offset
+// from a kernel argument keeps the loads in the global address space).
+struct EnsTile {
+  unsigned slot;             // the member's slot in the member table
+  unsigned pk;               // panel | tile column << 3 | tile row << 15 (tile_pack)
+  int target;                // steps the member takes in the current run
+  unsigned fstride;          // bytes from one array of the member to the next
+  long long fbase;           // the member's first array (Pref), from the member table
+  int n, h;
+};
+static_assert(sizeof(EnsTile) == 32, "one scalar load per tile record");
+
 __host__ __device__ constexpr unsigned tile_pack(int p, int bx, int by) {
   return static_cast<unsigned>(p) | (static_cast<unsigned>(bx) << 3) | (static_cast<unsigned>(by) << 15);
 }
 
 // One tile of the global list; false when its member has taken its steps
 // (members are ordered by decreasing step count, so every later tile of the
-// list is finished too).  The member's fields are read before the step test,
-// so their loads share one round trip.
+// list is finished too).  The member table's fields (constants, halo rules,
+// tendency slots) load while phase 1 runs.
 template <typename T, int KIND, int TY>
-__device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, const uint2* __restrict__ tiles,
+__device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, const EnsTile* __restrict__ tiles,
                                          const EnsStage& sel, const unsigned t, const int tid, const int step) {
-  const uint2 rec = tiles[t];
-  const EnsMember<T>* M = md + rec.x;
-  const int target = M->target;
+  // the whole record in one round trip (loaded as two 16-byte vectors, held
+  // until the step test: field by field, the compiler would wait for each)
+  const uint4* rv = reinterpret_cast<const uint4*>(tiles + t);
+  const uint4 r0 = rv[0], r1 = rv[1];
+  asm volatile("" ::"s"(r0.x), "s"(r0.y), "s"(r0.z), "s"(r0.w), "s"(r1.x), "s"(r1.y), "s"(r1.z), "s"(r1.w));
+  EnsTile rec;
+  rec.slot = r0.x;
+  rec.pk = r0.y;
+  rec.target = static_cast<int>(r0.z);
+  rec.fstride = r0.w;
+  rec.fbase = static_cast<long long>(static_cast<unsigned long long>(r1.x) | (static_cast<unsigned long long>(r1.y) << 32));
+  rec.n = static_cast<int>(r1.z);
+  rec.h = static_cast<int>(r1.w);
+  if (step >= rec.target) return false;
+  const EnsMember<T>* M = md + rec.slot;
+  char* const mbase = const_cast<char*>(reinterpret_cast<const char*>(md)) + rec.fbase;
+  auto arr = [&](int k) { return reinterpret_cast<T*>(mbase + static_cast<long long>(k) * rec.fstride); };
   SplitArgs<T> a = M->a;
-  a.in = Prog<const T>{M->buf[sel.in_b][0], M->buf[sel.in_b][1], M->buf[sel.in_b][2]};
-  a.base = Prog<const T>{M->buf[sel.base_b][0], M->buf[sel.base_b][1], M->buf[sel.base_b][2]};
-  a.out = Prog<T>{M->buf[sel.out_b][0], M->buf[sel.out_b][1], M->buf[sel.out_b][2]};
+  a.n = rec.n;
+  a.h = rec.h;
+  a.in = Prog<const T>{arr(4 + 3 * sel.in_b), arr(5 + 3 * sel.in_b), arr(6 + 3 * sel.in_b)};
+  a.base = Prog<const T>{arr(4 + 3 * sel.base_b), arr(5 + 3 * sel.base_b), arr(6 + 3 * sel.base_b)};
+  a.out = Prog<T>{arr(4 + 3 * sel.out_b), arr(5 + 3 * sel.out_b), arr(6 + 3 * sel.out_b)};
   for (int k = 0; k < 3; ++k) a.tend[k] = M->tend[sel.tend0 + k];
   for (int k = 0; k < 9; ++k) a.tacc[k] = M->tend[k];
   a.incs = sel.inc_full ? M->inc_full : M->inc_half;
-  const SplitStatics<T> st = M->st;
-  // the loads phase 1 depends on are issued with the step test's, in one
-  // round trip (the compiler would otherwise defer them past the test)
-  asm volatile("" ::"s"(target), "s"(a.n), "s"(a.h), "s"(a.in.phiG), "s"(a.in.u), "s"(a.in.v), "s"(st.pref),
-               "s"(st.sg), "s"(st.phis), "s"(rec.y));
-  if (step >= target) return false;
-  const int p = static_cast<int>(rec.y & 7u);
-  const int i0 = static_cast<int>((rec.y >> 3) & 0xFFFu) * TX;
-  const int j0 = static_cast<int>(rec.y >> 15) * TY;
+  const SplitStatics<T> st{arr(0), arr(1), arr(2), arr(3)};
+  const int p = static_cast<int>(rec.pk & 7u);
+  const int i0 = static_cast<int>((rec.pk >> 3) & 0xFFFu) * TX;
+  const int j0 = static_cast<int>(rec.pk >> 15) * TY;
   stage_tile<T, KIND, TY, 0>(a, st, M->hs[p], p, i0, j0, tid);
   return true;
 }
@@ -735,8 +759,8 @@ __device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, co
 // kernel starts after this one ends, and the launch that used them before
 // has ended.
 template <typename T, int KIND, int TY, bool PERSIST>
-__global__ void __launch_bounds__(TX * TY, (PERSIST && sizeof(T) == 4 && TY >= 16) ? 8 : 1) stage_ens(const EnsMember<T>* __restrict__ md,
-                                                      const uint2* __restrict__ tiles, int ntiles, EnsStage sel,
+__global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restrict__ md,
+                                                      const EnsTile* __restrict__ tiles, int ntiles, EnsStage sel,
                                                       const int* __restrict__ base, int koff,
                                                       unsigned* __restrict__ claim, unsigned* __restrict__ claim_next) {
   const unsigned nb = gridDim.x, b = blockIdx.x;
@@ -1355,7 +1379,7 @@ struct EnsPlan {
   int rebase_gx = 8;        // rebase grid: workgroups per member
   void* dmem = nullptr;     // member table, tile records, run step index, probe index, step counters
   void* d_desc = nullptr;
-  uint2* d_tiles = nullptr; // per tile: member slot, packed panel / tile column / tile row (tile_pack)
+  EnsTile* d_tiles = nullptr;  // per tile: member slot, tile position, the member's target and arrays
   int* d_run = nullptr;     // step index within the current run at the start of the graph being replayed
   int* d_probe = nullptr;   // -1: the step index of the autotuner's launches (every member steps)
   // persistent grid: the claim counters of kClaimSlots launches, used in
@@ -1367,17 +1391,43 @@ struct EnsPlan {
   int* d_done = nullptr;    // per member: steps since its upload, modulo kRebaseSteps
   std::vector<char> md_nat;           // member table in member order (EnsMember<T>)
   std::vector<int> tiles_per_member, nbx, nby;
+  std::vector<EnsTile> mtile;         // per member: the fields of its tile records that do not vary
   std::vector<int> target;            // targets of the table on the device (empty: none uploaded)
 };
 
 namespace {
 
-// The tile records of a member in table slot `slot`: x fastest, then y,
-// then panel (the order of its cells in the padded layout).
-void append_tiles(std::vector<uint2>& tiles, int slot, int nbx, int nby) {
+// The tile records of a member in table slot `slot` with `target` steps:
+// x fastest, then y, then panel (the order of its cells in the padded
+// layout).
+void append_tiles(std::vector<EnsTile>& tiles, const EnsTile& m, int slot, int target, int nbx, int nby) {
+  EnsTile r = m;
+  r.slot = static_cast<unsigned>(slot);
+  r.target = target;
   for (int p = 0; p < 6; ++p)
     for (int by = 0; by < nby; ++by)
-      for (int bx = 0; bx < nbx; ++bx) tiles.push_back(uint2{static_cast<unsigned>(slot), tile_pack(p, bx, by)});
+      for (int bx = 0; bx < nbx; ++bx) {
+        r.pk = tile_pack(p, bx, by);
+        tiles.push_back(r);
+      }
+}
+
+// The member's arrays at a fixed stride (EnsTile::fbase, here still the
+// address: the offset from the member table is taken once the table is
+// allocated), as split_create carves them; false if they are not.
+bool member_arrays(const SplitState* s, EnsTile* r) {
+  const char* b = static_cast<const char*>(s->pref);
+  const std::ptrdiff_t d = static_cast<const char*>(s->sg) - b;
+  if (d <= 0 || d >= (std::ptrdiff_t{1} << 32)) return false;
+  const void* arrs[13] = {s->pref, s->sg, s->phis, s->fs, s->buf[0][0], s->buf[0][1], s->buf[0][2],
+                          s->buf[1][0], s->buf[1][1], s->buf[1][2], s->buf[2][0], s->buf[2][1], s->buf[2][2]};
+  for (int k = 0; k < 13; ++k)
+    if (static_cast<const char*>(arrs[k]) != b + k * d) return false;
+  r->fbase = static_cast<long long>(reinterpret_cast<std::uintptr_t>(b));
+  r->fstride = static_cast<unsigned>(d);
+  r->n = s->n;
+  r->h = s->h;
+  return true;
 }
 
 constexpr int kClaimSlots = 6;
@@ -1444,10 +1494,11 @@ template <typename T>
 int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long long* done0, std::string* err) {
   const int TY = e->tile_y;
   std::vector<EnsMember<T>> md(nm);
-  std::vector<uint2> tiles;
+  std::vector<EnsTile> tiles;
   e->tiles_per_member.resize(nm);
   e->nbx.resize(nm);
   e->nby.resize(nm);
+  e->mtile.resize(nm);
   unsigned long long np_max = 0;
   for (int m = 0; m < nm; ++m) {
     const SplitState* s = mem[m];
@@ -1473,13 +1524,17 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long
     e->nbx[m] = d.a.nbx;
     e->nby[m] = d.a.nby;
     np_max = std::max(np_max, d.np);
-    append_tiles(tiles, m, d.a.nbx, d.a.nby);
+    if (!member_arrays(s, &e->mtile[m])) {
+      *err = "ensemble: a member's device arrays are not laid out at a fixed stride";
+      return CCS_ERR_INVALID;
+    }
+    append_tiles(tiles, e->mtile[m], m, 0, d.a.nbx, d.a.nby);
   }
   e->ntiles = static_cast<int>(tiles.size());
   // the rebase: about four cells per thread of the largest member
   e->rebase_gx = static_cast<int>(std::min<unsigned long long>(2048, std::max<unsigned long long>(8, (np_max + 1023) / 1024)));
   auto rup = [](std::size_t b) { return (b + 255) / 256 * 256; };
-  const std::size_t bd = rup(md.size() * sizeof(EnsMember<T>)), bt = rup(tiles.size() * sizeof(uint2));
+  const std::size_t bd = rup(md.size() * sizeof(EnsMember<T>)), bt = rup(tiles.size() * sizeof(EnsTile));
   const std::size_t bc = kClaimSlots * kClaimGroups * kClaimStride * sizeof(unsigned);
   const std::size_t bytes = bd + bt + 2 * rup(sizeof(int)) + rup(nm * sizeof(int)) + bc;
   if (hipMalloc(&e->dmem, bytes) != hipSuccess) {
@@ -1488,7 +1543,11 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long
   }
   char* p = static_cast<char*>(e->dmem);
   e->d_desc = p;
-  e->d_tiles = reinterpret_cast<uint2*>(p + bd);
+  // the records hold each member's arrays as an offset from the member table
+  const long long table = static_cast<long long>(reinterpret_cast<std::uintptr_t>(e->d_desc));
+  for (EnsTile& r : e->mtile) r.fbase -= table;
+  for (EnsTile& r : tiles) r.fbase -= table;
+  e->d_tiles = reinterpret_cast<EnsTile*>(p + bd);
   e->d_run = reinterpret_cast<int*>(p + bd + bt);
   e->d_probe = reinterpret_cast<int*>(p + bd + bt + rup(sizeof(int)));
   e->d_done = reinterpret_cast<int*>(p + bd + bt + 2 * rup(sizeof(int)));
@@ -1499,7 +1558,7 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long
   for (int m = 0; m < nm && done0; ++m) done[m] = static_cast<int>(done0[m] % kRebaseSteps);
   const int minus_one = -1, zero = 0;
   if (hipMemcpy(e->d_desc, md.data(), md.size() * sizeof(EnsMember<T>), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(EnsTile), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->d_run, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->d_probe, &minus_one, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->d_done, done.data(), nm * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1533,17 +1592,17 @@ int ens_set_run_t(EnsPlan* e, hipStream_t st, const int* target, std::string* er
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return target[x] > target[y]; });
   const auto* nat = reinterpret_cast<const EnsMember<T>*>(e->md_nat.data());
   std::vector<EnsMember<T>> md(nm);
-  std::vector<uint2> tiles;
+  std::vector<EnsTile> tiles;
   tiles.reserve(static_cast<std::size_t>(e->ntiles));
   for (int k = 0; k < nm; ++k) {
     const int m = order[k];
     md[k] = nat[m];
     md[k].target = target[m];
-    append_tiles(tiles, k, e->nbx[m], e->nby[m]);
+    append_tiles(tiles, e->mtile[m], k, target[m], e->nbx[m], e->nby[m]);
   }
   if (hipStreamSynchronize(st) != hipSuccess ||
       hipMemcpy(e->d_desc, md.data(), nm * sizeof(EnsMember<T>), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(EnsTile), hipMemcpyHostToDevice) != hipSuccess) {
     *err = "ensemble: run table upload failed";
     return CCS_ERR_DEVICE;
   }
