@@ -115,7 +115,7 @@ def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno", se
     wall = time.perf_counter() - t0
     ev_total, _ = core.time_steps(steps)            # HIP events around the graph replays
     nk = min(steps, 60)
-    _, kern_ms = core.time_steps(nk, kernel_timing=True)  # events around every stage kernel
+    _, kern_ms = core.time_steps(nk, kernel_timing=True)  # events attached to every stage dispatch
     core.advanced = 1 + settle + warmup + 2 * steps + 2 * nk  # time_steps(kernel_timing) runs two passes
     core.settle_steps = settle
     return core, wall, ev_total, kern_ms

@@ -146,9 +146,7 @@ int enqueue_step_f64(ccs_handle* hd, bool expl, hipEvent_t* ev) {
   static const char* const kStageName[4] = {"ccs stage 1", "ccs stage 2", "ccs stage 3", "ccs stage 4"};
   auto launch = [&](int kind, bool e) {
     const Range r(kStageName[stage]);
-    if (ev) (void)hipEventRecord(ev[2 * stage], hd->stream);
-    launch_stage_f64(hd->stream, kind, e, a, hd->st, hd->hs, hd->f64_tile_y);
-    if (ev) (void)hipEventRecord(ev[2 * stage + 1], hd->stream);
+    launch_stage_f64(hd->stream, kind, e, a, hd->st, hd->hs, hd->f64_tile_y, ev ? ev + 2 * stage : nullptr);
     ++stage;
   };
   if (hd->cfg.scheme == CCS_RK3_TVD) {
@@ -616,7 +614,8 @@ extern "C" int ccs_time_steps(ccs_handle* hd, int nsteps, double* total_ms, doub
   HIPCHK(hd, hipEventElapsedTime(&ms, ends[0], ends[1]));
   if (total_ms) *total_ms = ms;
   if (rc == CCS_OK && kernel_ms) {
-    // Second pass: events around every stage launch (no graph), mean duration.
+    // Second pass: every stage launch (no graph) carries its own start and end
+    // events (the dispatch's timestamps), mean duration.
     const int ns = stages_per_step(hd);
     EventSet ev(2 * ns);
     HIPCHK(hd, ev.create());

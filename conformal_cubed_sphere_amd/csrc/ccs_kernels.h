@@ -77,8 +77,9 @@ struct StageArgsF64 {
 // tile_y: rows per tile, 8 or 7 (stage_f64_tile_y: 7 where the tiles of an
 // n x n panel fit one round of workgroups on `cus` compute units)
 int stage_f64_tile_y(int n, int cus);
+// ev: NULL, or a start and an end event attached to the dispatch itself
 void launch_stage_f64(hipStream_t s, int kind, bool explicit_derived, const StageArgsF64& a,
-                      const StaticsF64& st, const HaloScatter& hs, int tile_y);
+                      const StaticsF64& st, const HaloScatter& hs, int tile_y, const hipEvent_t* ev = nullptr);
 
 // Derived fields for output on the padded grid (non-corner cells):
 // phi = phiG/sqrtG, (uc,vc) = IG (u,v); zonal/merid = M (u,v) on interior.

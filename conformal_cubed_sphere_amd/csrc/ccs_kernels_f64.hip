@@ -24,6 +24,8 @@
 // the cell and scatters edge-strip cells into the halos of the neighbouring
 // panels (the inverse of ghost_mod.f90:34-76), rotating the wind through the
 // destination panel's basis as convert_wind_SP2P does.
+#include <hip/hip_ext.h>
+
 #include "ccs_kernels.h"
 
 namespace ccs {
@@ -507,8 +509,17 @@ int stage_f64_tile_y(int n, int cus) {
   return wg7 <= 4L * cus ? 7 : 8;  // four workgroups fit a CU
 }
 
+// With an event pair, the events are the dispatch's own start and end
+// (hipExtLaunchKernelGGL), so they time the kernel alone.
+template <typename K>
+void launch_f64(K kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* ev, const StageArgsF64& a,
+                const StaticsF64& st, const HaloScatter& hs) {
+  if (ev) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev[0], ev[1], 0, a, st, hs);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, a, st, hs);
+}
+
 void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_in, const StaticsF64& st,
-                      const HaloScatter& hs, int tile_y) {
+                      const HaloScatter& hs, int tile_y, const hipEvent_t* ev) {
   StageArgsF64 a = a_in;
   a.rdx = 1.0 / a.dx;            // correctly rounded reciprocals for div_known
   a.r60dx = 1.0 / (60.0 * a.dx);
@@ -521,8 +532,8 @@ void launch_stage_f64(hipStream_t s, int kind, bool expl, const StageArgsF64& a_
   const dim3 block(NT);
 #define CCS_LAUNCH_TY(K, E, O)                                                           \
   do {                                                                                  \
-    if (ty == 7) hipLaunchKernelGGL((stage_f64<K, E, O, 7>), grid, block, 0, s, a, st, hs); \
-    else hipLaunchKernelGGL((stage_f64<K, E, O, 8>), grid, block, 0, s, a, st, hs);         \
+    if (ty == 7) launch_f64(stage_f64<K, E, O, 7>, grid, block, s, ev, a, st, hs); \
+    else launch_f64(stage_f64<K, E, O, 8>, grid, block, s, ev, a, st, hs);         \
   } while (0)
 #define CCS_LAUNCH_OP(K, O)                       \
   do {                                           \

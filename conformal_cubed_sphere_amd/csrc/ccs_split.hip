@@ -46,6 +46,7 @@
 // The centred operators of spatial_operators_mod_{2nd,4th,6th}.F90 are the
 // OP = 1/2/3 instantiations: no phase 2, and phase 3 takes centred
 // differences of the phase-1 fields.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <rocprofiler-sdk-roctx/roctx.h>
 
@@ -918,29 +919,37 @@ __global__ void __launch_bounds__(kDT) split_diag_final(const double* partials, 
 // and d below 2^11.
 unsigned tile_magic(unsigned d) { return static_cast<unsigned>((0x80000000ull + d - 1) / d); }
 
+// A stage launch; with an event pair, the events are the dispatch's own start
+// and end (hipExtLaunchKernelGGL), so they time the kernel alone.
+template <typename K, typename... Args>
+void launch_kernel(K kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* ev, Args... args) {
+  if (ev) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev[0], ev[1], 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
 template <typename T, int TY, int OP>
 void launch_split_op(hipStream_t s, int kind, const SplitArgs<T>& a, const SplitStatics<T>& st,
-                     const HaloScatter& hs) {
+                     const HaloScatter& hs, const hipEvent_t* ev) {
   const dim3 grid(static_cast<unsigned>(a.nbx * a.nby * 6));
   const dim3 block(TX * TY);
   switch (kind) {
-    case kTendOnly: hipLaunchKernelGGL((stage_split<T, kTendOnly, TY, OP>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s1: hipLaunchKernelGGL((stage_split<T, kRK3s1, TY, OP>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s2: hipLaunchKernelGGL((stage_split<T, kRK3s2, TY, OP>), grid, block, 0, s, a, st, hs); break;
-    case kRK3s3: hipLaunchKernelGGL((stage_split<T, kRK3s3, TY, OP>), grid, block, 0, s, a, st, hs); break;
-    case kRK4a: hipLaunchKernelGGL((stage_split<T, kRK4a, TY, OP>), grid, block, 0, s, a, st, hs); break;
-    default: hipLaunchKernelGGL((stage_split<T, kRK4f, TY, OP>), grid, block, 0, s, a, st, hs); break;
+    case kTendOnly: launch_kernel(stage_split<T, kTendOnly, TY, OP>, grid, block, s, ev, a, st, hs); break;
+    case kRK3s1: launch_kernel(stage_split<T, kRK3s1, TY, OP>, grid, block, s, ev, a, st, hs); break;
+    case kRK3s2: launch_kernel(stage_split<T, kRK3s2, TY, OP>, grid, block, s, ev, a, st, hs); break;
+    case kRK3s3: launch_kernel(stage_split<T, kRK3s3, TY, OP>, grid, block, s, ev, a, st, hs); break;
+    case kRK4a: launch_kernel(stage_split<T, kRK4a, TY, OP>, grid, block, s, ev, a, st, hs); break;
+    default: launch_kernel(stage_split<T, kRK4f, TY, OP>, grid, block, s, ev, a, st, hs); break;
   }
 }
 
 template <typename T, int TY>
 void launch_split_ty(hipStream_t s, int kind, int op, const SplitArgs<T>& a, const SplitStatics<T>& st,
-                     const HaloScatter& hs) {
+                     const HaloScatter& hs, const hipEvent_t* ev) {
   switch (op) {
-    case CCS_OP_CENTRED2: launch_split_op<T, TY, 1>(s, kind, a, st, hs); break;
-    case CCS_OP_CENTRED4: launch_split_op<T, TY, 2>(s, kind, a, st, hs); break;
-    case CCS_OP_CENTRED6: launch_split_op<T, TY, 3>(s, kind, a, st, hs); break;
-    default: launch_split_op<T, TY, 0>(s, kind, a, st, hs); break;
+    case CCS_OP_CENTRED2: launch_split_op<T, TY, 1>(s, kind, a, st, hs, ev); break;
+    case CCS_OP_CENTRED4: launch_split_op<T, TY, 2>(s, kind, a, st, hs, ev); break;
+    case CCS_OP_CENTRED6: launch_split_op<T, TY, 3>(s, kind, a, st, hs, ev); break;
+    default: launch_split_op<T, TY, 0>(s, kind, a, st, hs, ev); break;
   }
 }
 
@@ -948,14 +957,14 @@ void launch_split_ty(hipStream_t s, int kind, int op, const SplitArgs<T>& a, con
 // chosen one.  op: CCS_OP_WENO or a centred CCS_OP_CENTRED{2,4,6}.
 template <typename T>
 void launch_split(hipStream_t s, int kind, int op, SplitArgs<T> a, const SplitStatics<T>& st,
-                  const HaloScatter& hs, int tile_y) {
+                  const HaloScatter& hs, int tile_y, const hipEvent_t* ev = nullptr) {
   a.nbx = (a.n + TX - 1) / TX;
   a.nby = (a.n + tile_y - 1) / tile_y;
   a.mbx = tile_magic(static_cast<unsigned>(a.nbx));
   a.mby = tile_magic(static_cast<unsigned>(a.nby));
-  if (tile_y == 16) launch_split_ty<T, 16>(s, kind, op, a, st, hs);
-  else if (tile_y == 32) launch_split_ty<T, 32>(s, kind, op, a, st, hs);
-  else launch_split_ty<T, 8>(s, kind, op, a, st, hs);
+  if (tile_y == 16) launch_split_ty<T, 16>(s, kind, op, a, st, hs, ev);
+  else if (tile_y == 32) launch_split_ty<T, 32>(s, kind, op, a, st, hs, ev);
+  else launch_split_ty<T, 8>(s, kind, op, a, st, hs, ev);
 }
 
 }  // namespace
@@ -1022,9 +1031,7 @@ void enqueue_step_t(SplitState* s, hipStream_t st, int scheme, hipEvent_t* ev) {
   static const char* const kStageName[4] = {"ccs stage 1", "ccs stage 2", "ccs stage 3", "ccs stage 4"};
   auto launch = [&](int kind) {
     roctxRangePush(kStageName[stage]);
-    if (ev) (void)hipEventRecord(ev[2 * stage], st);
-    launch_split<T>(st, kind, s->cfg.spatial_op, a, S, s->hs, s->tile_y);
-    if (ev) (void)hipEventRecord(ev[2 * stage + 1], st);
+    launch_split<T>(st, kind, s->cfg.spatial_op, a, S, s->hs, s->tile_y, ev ? ev + 2 * stage : nullptr);
     roctxRangePop();
     ++stage;
   };

@@ -226,7 +226,8 @@ int ccs_diagnostics(ccs_handle* h, double* total_mass, double* total_energy);
 
 /* Time nsteps steps with HIP events on the handle's stream.  total_ms: wall
  * time of the whole loop; kernel_ms (optional): mean duration of one stage
- * kernel measured with events around every stage launch in a second pass. */
+ * kernel in a second pass of nsteps steps, each stage launched with a start
+ * and an end event attached to its dispatch (the kernel's own timestamps). */
 int ccs_time_steps(ccs_handle* h, int nsteps, double* total_ms,
                    double* kernel_ms);
 
