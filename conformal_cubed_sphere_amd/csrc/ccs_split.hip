@@ -694,14 +694,15 @@ __device__ __forceinline__ int run_step(const int* __restrict__ base, int k) { r
 // workgroup issues its tile's loads after one load that depends on nothing
 // but its index.  A member's 13 padded arrays Pref, sqrtg, phis, a sqrtg f
 // and the three state buffers (dP, U, V) follow each other at a fixed
-// stride; the first lies `fbase` bytes from the member table (an offset
-// from a kernel argument keeps the loads in the global address space).
+// stride; the first lies `fbase` bytes from the kernel's `arena` argument (an
+// offset from a kernel argument keeps the accesses in the global address
+// space).
 struct EnsTile {
   unsigned slot;             // the member's slot in the member table
   unsigned pk;               // panel | tile column << 3 | tile row << 15 (tile_pack)
   int target;                // steps the member takes in the current run
   unsigned fstride;          // bytes from one array of the member to the next
-  long long fbase;           // the member's first array (Pref), from the member table
+  long long fbase;           // the member's first array (Pref), from the arena argument
   int n, h;
 };
 static_assert(sizeof(EnsTile) == 32, "one scalar load per tile record");
@@ -716,7 +717,8 @@ __host__ __device__ constexpr unsigned tile_pack(int p, int bx, int by) {
 // tendency slots) load while phase 1 runs.
 template <typename T, int KIND, int TY>
 __device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, const EnsTile* __restrict__ tiles,
-                                         const EnsStage& sel, const unsigned t, const int tid, const int step) {
+                                         char* arena, const EnsStage& sel, const unsigned t, const int tid,
+                                         const int step) {
   // the whole record in one round trip (loaded as two 16-byte vectors, held
   // until the step test: field by field, the compiler would wait for each)
   const uint4* rv = reinterpret_cast<const uint4*>(tiles + t);
@@ -732,7 +734,7 @@ __device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, co
   rec.h = static_cast<int>(r1.w);
   if (step >= rec.target) return false;
   const EnsMember<T>* M = md + rec.slot;
-  char* const mbase = const_cast<char*>(reinterpret_cast<const char*>(md)) + rec.fbase;
+  char* const mbase = arena + rec.fbase;
   auto arr = [&](int k) { return reinterpret_cast<T*>(mbase + static_cast<long long>(k) * rec.fstride); };
   SplitArgs<T> a = M->a;
   a.n = rec.n;
@@ -761,13 +763,14 @@ __device__ __forceinline__ bool ens_tile(const EnsMember<T>* __restrict__ md, co
 // has ended.
 template <typename T, int KIND, int TY, bool PERSIST>
 __global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restrict__ md,
-                                                      const EnsTile* __restrict__ tiles, int ntiles, EnsStage sel,
+                                                      const EnsTile* __restrict__ tiles, char* arena, int ntiles,
+                                                      EnsStage sel,
                                                       const int* __restrict__ base, int koff,
                                                       unsigned* __restrict__ claim, unsigned* __restrict__ claim_next) {
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const int step = run_step(base, koff);
   if (!PERSIST) {
-    (void)ens_tile<T, KIND, TY>(md, tiles, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x), step);
+    (void)ens_tile<T, KIND, TY>(md, tiles, arena, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x), step);
     return;
   }
   if (b == 0 && threadIdx.x < kClaimGroups) claim_next[threadIdx.x * kClaimStride] = 0u;
@@ -787,7 +790,7 @@ __global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restr
     // hold ~30 more VGPRs for the whole kernel and halve the occupancy
     int tid = static_cast<int>(threadIdx.x);
     asm volatile("" : "+v"(tid));
-    if (!ens_tile<T, KIND, TY>(md, tiles, sel, t, tid, step)) break;
+    if (!ens_tile<T, KIND, TY>(md, tiles, arena, sel, t, tid, step)) break;
     if (threadIdx.x == 0) s_tile[(k + 1u) & 1u] = nxt;
   }
 }
@@ -1420,8 +1423,9 @@ void append_tiles(std::vector<EnsTile>& tiles, const EnsTile& m, int slot, int t
 }
 
 // The member's arrays at a fixed stride (EnsTile::fbase, here still the
-// address: the offset from the member table is taken once the table is
-// allocated), as split_create carves them; false if they are not.
+// address: the offset from the arena, the member table's allocation, is
+// taken once that is allocated), as split_create carves them; false if they
+// are not.
 bool member_arrays(const SplitState* s, EnsTile* r) {
   const char* b = static_cast<const char*>(s->pref);
   const std::ptrdiff_t d = static_cast<const char*>(s->sg) - b;
@@ -1455,12 +1459,12 @@ void ens_launch_k(const EnsPlan* e, hipStream_t s, int kind, const EnsStage& sel
   const dim3 grid(static_cast<unsigned>(e->grid)), block(TX * TY);
   const int nt = e->ntiles;
   switch (kind) {
-    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
-    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
-    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
-    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
-    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
-    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, nt, sel, run, koff, c.cur, c.next); break;
+    case kTendOnly: hipLaunchKernelGGL((stage_ens<T, kTendOnly, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, static_cast<char*>(e->d_desc), nt, sel, run, koff, c.cur, c.next); break;
+    case kRK3s1: hipLaunchKernelGGL((stage_ens<T, kRK3s1, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, static_cast<char*>(e->d_desc), nt, sel, run, koff, c.cur, c.next); break;
+    case kRK3s2: hipLaunchKernelGGL((stage_ens<T, kRK3s2, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, static_cast<char*>(e->d_desc), nt, sel, run, koff, c.cur, c.next); break;
+    case kRK3s3: hipLaunchKernelGGL((stage_ens<T, kRK3s3, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, static_cast<char*>(e->d_desc), nt, sel, run, koff, c.cur, c.next); break;
+    case kRK4a: hipLaunchKernelGGL((stage_ens<T, kRK4a, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, static_cast<char*>(e->d_desc), nt, sel, run, koff, c.cur, c.next); break;
+    default: hipLaunchKernelGGL((stage_ens<T, kRK4f, TY, PERSIST>), grid, block, 0, s, md, e->d_tiles, static_cast<char*>(e->d_desc), nt, sel, run, koff, c.cur, c.next); break;
   }
 }
 
@@ -1550,7 +1554,8 @@ int ens_build(EnsPlan* e, SplitState* const* mem, int nm, int device, const long
   }
   char* p = static_cast<char*>(e->dmem);
   e->d_desc = p;
-  // the records hold each member's arrays as an offset from the member table
+  // the records hold each member's arrays as an offset from the arena (the
+  // member table's address, passed to the stage kernel as a plain pointer)
   const long long table = static_cast<long long>(reinterpret_cast<std::uintptr_t>(e->d_desc));
   for (EnsTile& r : e->mtile) r.fbase -= table;
   for (EnsTile& r : tiles) r.fbase -= table;
