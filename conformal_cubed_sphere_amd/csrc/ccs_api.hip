@@ -784,7 +784,8 @@ void tune_store(const std::string& key, const TuneChoice& c) {
   tune_load_locked();
   g_tune[key] = c;
   const char* path = std::getenv("CCS_AUTOTUNE_CACHE");
-  if (!path || !*path) return;
+  // keys the loader's 4095-character field cannot hold stay in memory only
+  if (!path || !*path || key.size() > 4000) return;
   if (std::FILE* f = std::fopen(path, "a")) {
     std::fprintf(f, "%s %d %d %.9g\n", key.c_str(), c.tile_y, c.persistent, c.stage_ms);
     std::fclose(f);
