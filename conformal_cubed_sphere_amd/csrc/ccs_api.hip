@@ -50,6 +50,7 @@ struct ccs_handle {
   double* result = nullptr;
   double* face_speed = nullptr;  // CCS_OP_WENO_EXT: x-face then y-face G^-1 entries
   int f64_tile_y = 8;            // rows per tile of the FP64 stage kernel (stage_f64_tile_y)
+  unsigned launches = 0;         // FP64 stage launches enqueued (their parity sets the tile order)
   // split-precision path
   SplitState* split = nullptr;
   // Step graphs.  Every step leaves the state in the same buffer, so a graph
@@ -146,6 +147,7 @@ int enqueue_step_f64(ccs_handle* hd, bool expl, hipEvent_t* ev) {
   static const char* const kStageName[4] = {"ccs stage 1", "ccs stage 2", "ccs stage 3", "ccs stage 4"};
   auto launch = [&](int kind, bool e) {
     const Range r(kStageName[stage]);
+    a.rev = hd->launches++ & 1u;  // alternate the tile order (StageArgsF64::rev)
     launch_stage_f64(hd->stream, kind, e, a, hd->st, hd->hs, hd->f64_tile_y, ev ? ev + 2 * stage : nullptr);
     ++stage;
   };

@@ -72,6 +72,11 @@ struct StageArgsF64 {
   // y-face (i, j+1/2) for i = 1..n, j = 0..n: [i-1 + n(j + (n+1) p)]
   const double* igx;
   const double* igy;
+  // 1: this launch takes the tiles in reverse order.  Consecutive stage
+  // launches alternate, so the tiles one launch reaches last (the most recent
+  // lines in the memory-side cache) are the first the next launch reads.
+  // Results do not depend on the order.
+  unsigned rev;
 };
 
 // tile_y: rows per tile, 8 or 7 (stage_f64_tile_y: 7 where the tiles of an

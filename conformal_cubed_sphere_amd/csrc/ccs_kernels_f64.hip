@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
   const unsigned nbx = (static_cast<unsigned>(a.n) + TX - 1) / TX, nby = (static_cast<unsigned>(a.n) + TY - 1) / TY;
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const unsigned xcd = b & 7u;
-  const unsigned t = xcd * (nb >> 3) + min(xcd, nb & 7u) + (b >> 3);
+  const unsigned tf = xcd * (nb >> 3) + min(xcd, nb & 7u) + (b >> 3);
+  const unsigned t = a.rev ? nb - 1u - tf : tf;  // alternating order (StageArgsF64::rev)
   const unsigned tq = t / nbx;
   const int p = static_cast<int>(tq / nby);
   const int i0 = static_cast<int>(t - tq * nbx) * TX, j0 = static_cast<int>(tq - static_cast<unsigned>(p) * nby) * TY;

@@ -112,6 +112,7 @@ struct SplitArgs {
   // WENO-Z epsilons of Fx/Fy, E (windows scaled by 1/2: times 1/16) and of
   // phitG/a^2 (scaled by 1/4: times 1/64)
   T epsF, epsE, epsP;
+  unsigned rev;        // 1: this launch takes the tiles in reverse order (xcd_tile_dir)
 };
 
 // One 5-point window (q0..q4, centre q2): the WENO-Z value at the centre
@@ -263,6 +264,17 @@ __device__ __forceinline__ T d6f(T m3, T m2, T m1, T p1, T p2, T p3, T rdx) {
 __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   const unsigned xq = nb >> 3, xr = nb & 7u, xcd = b & 7u;
   return xcd * xq + min(xcd, xr) + (b >> 3);
+}
+
+// The same order, reversed when rev is set.  Consecutive stage launches
+// alternate: the tiles one launch reaches last, whose outputs and statics are
+// the most recent lines in the memory-side cache, are the first the next
+// launch reads.  Results do not depend on the order (every cell and halo cell
+// is written by one thread of one tile); it pays where a stage's working set
+// exceeds that cache (C1440, large ensembles: about 3 %).
+__device__ __forceinline__ unsigned xcd_tile_dir(unsigned b, unsigned nb, unsigned rev) {
+  const unsigned t = xcd_tile(b, nb);
+  return rev ? nb - 1u - t : t;
 }
 
 // One RK stage on one 32 x TY tile of panel p with lower-left interior cell
@@ -650,7 +662,7 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
 
 template <typename T, int KIND, int TY, int OP>
 __global__ void __launch_bounds__(TX * TY) stage_split(SplitArgs<T> a, SplitStatics<T> st, HaloScatter hs) {
-  const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned t = xcd_tile_dir(blockIdx.x, gridDim.x, a.rev);
   const unsigned tq = __umulhi(t << 1, a.mbx);
   const int p = static_cast<int>(__umulhi(tq << 1, a.mby));
   const int i0 = static_cast<int>(t - tq * a.nbx) * TX;
@@ -681,6 +693,7 @@ struct EnsStage {
   int in_b, base_b, out_b;
   int tend0;    // first tendency slot written (RK4 stages a)
   int inc_full; // RK4: 1 -> dt/a increment, 0 -> dt/(2a)
+  int rev;      // 1: tiles in reverse order (xcd_tile_dir; one workgroup per tile only)
 };
 
 // Step index within the current run: the index at the start of the graph
@@ -770,7 +783,8 @@ __global__ void __launch_bounds__(TX * TY) stage_ens(const EnsMember<T>* __restr
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const int step = run_step(base, koff);
   if (!PERSIST) {
-    (void)ens_tile<T, KIND, TY>(md, tiles, arena, sel, xcd_tile(b, nb), static_cast<int>(threadIdx.x), step);
+    (void)ens_tile<T, KIND, TY>(md, tiles, arena, sel, xcd_tile_dir(b, nb, static_cast<unsigned>(sel.rev)),
+                                static_cast<int>(threadIdx.x), step);
     return;
   }
   if (b == 0 && threadIdx.x < kClaimGroups) claim_next[threadIdx.x * kClaimStride] = 0u;
@@ -979,6 +993,7 @@ struct SplitState {
   bool fp64 = false;
   int n = 0, h = 0;
   int tile_y = 16;  // stage-kernel tile height (8, 16 or 32)
+  unsigned launches = 0;  // stage launches enqueued (their parity sets the tile order)
   std::size_t np = 0, ni = 0, esz = 4;
   void* dmem = nullptr;
   // typed device arrays (float or double)
@@ -1033,6 +1048,7 @@ void enqueue_step_t(SplitState* s, hipStream_t st, int scheme, hipEvent_t* ev) {
   int stage = 0;
   static const char* const kStageName[4] = {"ccs stage 1", "ccs stage 2", "ccs stage 3", "ccs stage 4"};
   auto launch = [&](int kind) {
+    a.rev = s->launches++ & 1u;  // alternate the tile order (xcd_tile_dir)
     roctxRangePush(kStageName[stage]);
     launch_split<T>(st, kind, s->cfg.spatial_op, a, S, s->hs, s->tile_y, ev ? ev + 2 * stage : nullptr);
     roctxRangePop();
@@ -1631,14 +1647,16 @@ void ens_steps_t(const EnsPlan* e, hipStream_t st, int scheme, int steps, unsign
   const auto* md = static_cast<const EnsMember<T>*>(e->d_desc);
   for (int k = 0; k < steps; ++k) {
     if (scheme == CCS_RK3_TVD) {
-      ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0}, run, k, 0, 3);
-      ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0}, run, k, 1, 3);
-      ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0}, run, k, 2, 3);
+      // tile order alternating from launch to launch (xcd_tile_dir)
+      const int r = (3 * k) & 1;
+      ens_launch<T>(e, st, kRK3s1, EnsStage{0, 0, 1, 0, 0, r}, run, k, 0, 3);
+      ens_launch<T>(e, st, kRK3s2, EnsStage{1, 0, 2, 0, 0, r ^ 1}, run, k, 1, 3);
+      ens_launch<T>(e, st, kRK3s3, EnsStage{2, 0, 0, 0, 0, r}, run, k, 2, 3);
     } else {
-      ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0}, run, k, 0, 4);
-      ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0}, run, k, 1, 4);
-      ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1}, run, k, 2, 4);
-      ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0}, run, k, 3, 4);
+      ens_launch<T>(e, st, kRK4a, EnsStage{0, 0, 1, 0, 0, 0}, run, k, 0, 4);
+      ens_launch<T>(e, st, kRK4a, EnsStage{1, 0, 2, 3, 0, 1}, run, k, 1, 4);
+      ens_launch<T>(e, st, kRK4a, EnsStage{2, 0, 1, 6, 1, 0}, run, k, 2, 4);
+      ens_launch<T>(e, st, kRK4f, EnsStage{1, 0, 0, 0, 0, 1}, run, k, 3, 4);
     }
     if ((rebase_mask >> k) & 1u)
       hipLaunchKernelGGL(ens_rebase<T>, dim3(static_cast<unsigned>(e->rebase_gx), static_cast<unsigned>(e->nm)),
