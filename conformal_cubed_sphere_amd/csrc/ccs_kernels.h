@@ -72,10 +72,10 @@ struct StageArgsF64 {
   // y-face (i, j+1/2) for i = 1..n, j = 0..n: [i-1 + n(j + (n+1) p)]
   const double* igx;
   const double* igy;
-  // 1: this launch takes the tiles in reverse order.  Consecutive stage
-  // launches alternate, so the tiles one launch reaches last (the most recent
-  // lines in the memory-side cache) are the first the next launch reads.
-  // Results do not depend on the order.
+  // 1: each XCD takes its contiguous run of tiles backwards.  Consecutive
+  // stage launches alternate, so every XCD stays on its own run (its L2) and
+  // starts on the tiles it reached last (the most recent lines in the
+  // memory-side cache).  Results do not depend on the order.
   unsigned rev;
 };
 

@@ -138,7 +138,9 @@ __global__ void __launch_bounds__(NT) stage_f64(StageArgsF64 a, StaticsF64 st, H
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const unsigned xcd = b & 7u;
   const unsigned tf = xcd * (nb >> 3) + min(xcd, nb & 7u) + (b >> 3);
-  const unsigned t = a.rev ? nb - 1u - tf : tf;  // alternating order (StageArgsF64::rev)
+  // rev: this XCD's run of tiles backwards (StageArgsF64::rev)
+  const unsigned len = (nb >> 3) + (xcd < (nb & 7u) ? 1u : 0u);
+  const unsigned t = a.rev ? tf - (b >> 3) + (len - 1u - (b >> 3)) : tf;
   const unsigned tq = t / nbx;
   const int p = static_cast<int>(tq / nby);
   const int i0 = static_cast<int>(t - tq * nbx) * TX, j0 = static_cast<int>(tq - static_cast<unsigned>(p) * nby) * TY;

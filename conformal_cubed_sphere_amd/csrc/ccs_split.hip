@@ -112,7 +112,7 @@ struct SplitArgs {
   // WENO-Z epsilons of Fx/Fy, E (windows scaled by 1/2: times 1/16) and of
   // phitG/a^2 (scaled by 1/4: times 1/64)
   T epsF, epsE, epsP;
-  unsigned rev;        // 1: this launch takes the tiles in reverse order (xcd_tile_dir)
+  unsigned rev;        // 1: each XCD takes its run of tiles backwards (xcd_tile_dir)
 };
 
 // One 5-point window (q0..q4, centre q2): the WENO-Z value at the centre
@@ -266,15 +266,19 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   return xcd * xq + min(xcd, xr) + (b >> 3);
 }
 
-// The same order, reversed when rev is set.  Consecutive stage launches
-// alternate: the tiles one launch reaches last, whose outputs and statics are
-// the most recent lines in the memory-side cache, are the first the next
-// launch reads.  Results do not depend on the order (every cell and halo cell
-// is written by one thread of one tile); it pays where a stage's working set
-// exceeds that cache (C1440, large ensembles: about 3 %).
+// The same order with each XCD's run of tiles taken backwards when rev is
+// set.  Consecutive stage launches alternate, so every XCD stays on its own
+// run (what its L2 may still hold helps small grids) and starts on the tiles
+// it reached last, whose outputs and statics are the most recent lines in the
+// memory-side cache (that helps where a stage's working set exceeds the
+// cache: C1440 and large ensembles, about 3-4 %).  Results do not depend on
+// the order: every cell and halo cell is written by one thread of one tile.
+// (Reversing the whole list instead put each XCD on another XCD's run and
+// cost 10 % at C180 in FP32.)
 __device__ __forceinline__ unsigned xcd_tile_dir(unsigned b, unsigned nb, unsigned rev) {
-  const unsigned t = xcd_tile(b, nb);
-  return rev ? nb - 1u - t : t;
+  const unsigned xq = nb >> 3, xr = nb & 7u, xcd = b & 7u;
+  const unsigned len = xq + (xcd < xr ? 1u : 0u), j = b >> 3;  // workgroup b is number j of len on its XCD
+  return xcd * xq + min(xcd, xr) + (rev ? len - 1u - j : j);
 }
 
 // One RK stage on one 32 x TY tile of panel p with lower-left interior cell
@@ -693,7 +697,7 @@ struct EnsStage {
   int in_b, base_b, out_b;
   int tend0;    // first tendency slot written (RK4 stages a)
   int inc_full; // RK4: 1 -> dt/a increment, 0 -> dt/(2a)
-  int rev;      // 1: tiles in reverse order (xcd_tile_dir; one workgroup per tile only)
+  int rev;      // 1: each XCD's run of tiles backwards (xcd_tile_dir; one workgroup per tile only)
 };
 
 // Step index within the current run: the index at the start of the graph
