@@ -76,6 +76,14 @@ __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x);
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+// a * b + c with one rounding.  The face values and Rusanov speeds are written
+// with explicit FMAs and no implicit contraction: the tile-boundary windows
+// (weno_side) and the interior ones (weno_faces), the two copies of the
+// pointwise terms (a tile's own cells and its border) and every instance of
+// lam_x / lam_y then round alike, so a face value does not depend on which
+// tile computes it, and results do not depend on the tile height.
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Loads and stores at a 32-bit byte offset from a uniform base pointer: the
 // compiler then addresses every array of a cell with one shared VGPR offset
@@ -157,22 +165,22 @@ struct WenoWin {
 // are the window's values times 1/(4 SC).
 template <typename T, int SC>
 __device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T q4, T eps_scaled) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
   WenoWin<T, SC> w;
   const T d0 = q0 - q2, d1 = q1 - q2, d3 = q3 - q2, d4 = q4 - q2;
-  w.a1 = T(-2) * d1 + d0;
+  w.a1 = fmat(T(-2), d1, d0);
   w.a2 = d1 + d3;
-  w.a3 = T(-2) * d3 + d4;
+  w.a3 = fmat(T(-2), d3, d4);
   // b1 = a1 - 2 d1, b2 = d1 - d3, b3 = a3 - 2 d3 (times 1/(4 SC) with the inputs)
-  w.q1 = T(-2) * d1 + w.a1;
+  w.q1 = fmat(T(-2), d1, w.a1);
   w.q2 = d1 - d3;
-  w.q3 = T(-2) * d3 + w.a3;
+  w.q3 = fmat(T(-2), d3, w.a3);
   // beta_k / (2 SC)^2 = 13/12 a_k^2 / (2 SC)^2 + (b_k / (4 SC))^2; with the
   // scaled a_k that is 13/3 a_k^2 + q_k^2
   const T k13 = T(13) / T(3);
-  const T be1 = (k13 * w.a1) * w.a1 + w.q1 * w.q1;
-  const T be2 = (k13 * w.a2) * w.a2 + w.q2 * w.q2;
-  const T be3 = (k13 * w.a3) * w.a3 + w.q3 * w.q3;
+  const T be1 = fmat(k13 * w.a1, w.a1, w.q1 * w.q1);
+  const T be2 = fmat(k13 * w.a2, w.a2, w.q2 * w.q2);
+  const T be3 = fmat(k13 * w.a3, w.a3, w.q3 * w.q3);
   w.t40 = fabs(be1 - be2);
   w.t41 = fabs(be2 - be3);
   const T t5 = fabs(be3 - be1);
@@ -181,9 +189,9 @@ __device__ __forceinline__ WenoWin<T, SC> weno_window(T q0, T q1, T q2, T q3, T 
   // the factors 3 and 6 multiply, since fma cannot take them as an inline
   // addend
   const T r1 = rcp(eps_scaled + be1), r2 = rcp(eps_scaled + be2), r3 = rcp(eps_scaled + be3);
-  w.u1 = t5 * r1 + T(1);
-  w.v3 = t5 * r3 + T(1);
-  w.u2 = T(6) * (t5 * r2 + T(1));
+  w.u1 = fmat(t5, r1, T(1));
+  w.v3 = fmat(t5, r3, T(1));
+  w.u2 = T(6) * fmat(t5, r2, T(1));
   w.u3 = T(3) * w.v3;
   w.v1 = T(3) * w.u1;
   return w;
@@ -212,15 +220,15 @@ __device__ __forceinline__ void weno_branch_weights(T t40, T t41, T mb, T& w1, T
 // normalised combination of the three candidates
 template <typename T>
 __device__ __forceinline__ T weno_combine(T s1, T s2, T s3, T w1, T w2, T w3) {
-#pragma clang fp contract(fast)
-  return (w1 * s1 + w2 * s2 + w3 * s3) * rcp(w1 + w2 + w3);
+#pragma clang fp contract(off)
+  return fmat(w3, s3, fmat(w2, s2, w1 * s1)) * rcp(w1 + w2 + w3);
 }
 
 // values at the right and the left face of the window's centre cell (the
 // left from the mirrored window), times 1/SC
 template <typename T, int SC>
 __device__ __forceinline__ void weno_faces(const WenoWin<T, SC>& w, T& right, T& left) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
   constexpr T kc = T(1) / T(3);  // a_k / (12 SC) of the inputs scaled by 1/(4 SC)
   T r1 = w.u1, r2 = w.u2, r3 = w.u3, l1 = w.v3, l2 = w.u2, l3 = w.v1;
   if ((w.t40 <= w.mb) != (w.t41 <= w.mb)) {
@@ -230,18 +238,19 @@ __device__ __forceinline__ void weno_faces(const WenoWin<T, SC>& w, T& right, T&
     weno_branch_weights(w.t40, w.t41, w.mb, r1, r2, r3);
     weno_branch_weights(w.t41, w.t40, w.mb, l1, l2, l3);
   }
-  right = weno_combine(w.a1 * kc + w.q1, w.a2 * kc - w.q2, w.a3 * kc - w.q3, r1, r2, r3);
-  left = weno_combine(w.a3 * kc + w.q3, w.a2 * kc + w.q2, w.a1 * kc - w.q1, l1, l2, l3);
+  right = weno_combine(fmat(w.a1, kc, w.q1), fmat(w.a2, kc, -w.q2), fmat(w.a3, kc, -w.q3), r1, r2, r3);
+  left = weno_combine(fmat(w.a3, kc, w.q3), fmat(w.a2, kc, w.q2), fmat(w.a1, kc, -w.q1), l1, l2, l3);
 }
 
 // either face, chosen per lane (left = true: the left face), times 1/SC
 template <typename T, int SC>
 __device__ __forceinline__ T weno_side(const WenoWin<T, SC>& w, bool left) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
   constexpr T kc = T(1) / T(3);
-  const T s1 = left ? w.a3 * kc + w.q3 : w.a1 * kc + w.q1;
-  const T s2 = w.a2 * kc + (left ? w.q2 : -w.q2);
-  const T s3 = left ? w.a1 * kc - w.q1 : w.a3 * kc - w.q3;
+  // the candidates of weno_faces, operand for operand
+  const T s1 = left ? fmat(w.a3, kc, w.q3) : fmat(w.a1, kc, w.q1);
+  const T s2 = fmat(w.a2, kc, left ? w.q2 : -w.q2);
+  const T s3 = left ? fmat(w.a1, kc, -w.q1) : fmat(w.a3, kc, -w.q3);
   T w1 = left ? w.v3 : w.u1, w2 = w.u2, w3 = left ? w.v1 : w.u3;
   if ((w.t40 <= w.mb) != (w.t41 <= w.mb)) {
     asm volatile("");
@@ -360,6 +369,7 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
     return r;
   };
   auto pointwise = [&](const Raw& r) {
+#pragma clang fp contract(off)
     if (r.l < 0) return;
     const T Pv = r.pr + r.dp;
     const T U = r.u, V = r.v;
@@ -369,9 +379,9 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
     // uploaded times kF): the contravariant winds carry the factor kF
     const T rgc = rg * kF;
     const T Uc = U * rgc, Vc = V * rgc;
-    const T phitc = Pv * rgc + r.ps;
+    const T phitc = fmat(Pv, rgc, r.ps);
     const int l = r.l;
-    sE[l] = phitc + T(0.5) * (U * Uc + V * Vc);
+    sE[l] = fmat(T(0.5), fmat(U, Uc, V * Vc), phitc);
     sFx[l] = Pv * Uc;
     sFy[l] = Pv * Vc;
     sPt[l] = phitc * (r.sg * (kP / kF));
@@ -407,16 +417,18 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
   // phitG carries kP = 1/16, so its square root a factor 1/4).
   constexpr T kRootP = T(4);
   auto lam_x = [&](int l) {
+#pragma clang fp contract(off)
     const T r0 = sRg[l], r1 = sRg[l + 1];
-    const T ub = sU[l] * r0 + sU[l + 1] * r1;
-    const T pb = sPt[l] * r0 + sPt[l + 1] * r1;
-    return fsqrt((r0 + r1) * pb) * kRootP + fabs(ub);
+    const T ub = fmat(sU[l], r0, sU[l + 1] * r1);
+    const T pb = fmat(sPt[l], r0, sPt[l + 1] * r1);
+    return fmat(fsqrt((r0 + r1) * pb), kRootP, static_cast<T>(fabs(ub)));
   };
   auto lam_y = [&](int l) {
+#pragma clang fp contract(off)
     const T r0 = sRg[l], r1 = sRg[l + EX];
-    const T vb = sV[l] * r0 + sV[l + EX] * r1;
-    const T pb = sPt[l] * r0 + sPt[l + EX] * r1;
-    return fsqrt((r0 + r1) * pb) * kRootP + fabs(vb);
+    const T vb = fmat(sV[l], r0, sV[l + EX] * r1);
+    const T pb = fmat(sPt[l], r0, sPt[l + EX] * r1);
+    return fmat(fsqrt((r0 + r1) * pb), kRootP, static_cast<T>(fabs(vb)));
   };
 
   // ---- phase 2: WENO windows and face shares.  Thread (tx, ty) owns the
@@ -473,8 +485,8 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       weno_faces(wp, pR, pL);
       weno_faces(we, er, el);
       // (F + lam P)/2 = F/2 + (2 lam) (P/4)
-      fr = fR + lr * pR;
-      fl = fL - ll * pL;
+      fr = fmat(lr, pR, fR);
+      fl = fmat(-ll, pL, fL);
       xpF[fx + 1] = fr;
       xmF[fx] = fl;
       xpE[fx + 1] = er;
@@ -490,8 +502,8 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
       weno_faces(wf, fR, fL);
       weno_faces(wp, pR, pL);
       weno_faces(we, eu, ed);
-      fu = fR + lu * pR;
-      fd = fL - ld * pL;
+      fu = fmat(lu, pR, fR);
+      fd = fmat(-ld, pL, fL);
       ypF[fy + TX] = fu;
       ymF[fy] = fd;
       ypE[fy + TX] = eu;
@@ -544,7 +556,7 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
         const T lam = (q < NXE) ? lam_x(lf) : lam_y(lf);  // twice the speed
         const T vp = weno_side(wp, left), vf = weno_side(wf, left);
         T* oF = (q < NXE) ? (left ? xmF : xpF) : (left ? ymF : ypF);
-        oF[f] = left ? vf - lam * vp : vf + lam * vp;
+        oF[f] = fmat(left ? -lam : lam, vp, vf);  // as fr, fl of the interior windows
       }
     }
     pre = preload();

@@ -53,7 +53,10 @@ def test_members_equal_single_runs(ccs, precision, scheme, persistent):
             assert np.isfinite(mid[k][0])
 
 
-def test_autotuned_members_equal_single_runs_at_the_chosen_tile(ccs, monkeypatch):
+def test_autotuned_members_equal_single_runs(ccs):
+    # whatever tile height the autotuner picks, members equal the single runs
+    # (which use 16-row tiles): results do not depend on the tile height
+    assert "CCS_SPLIT_TILE_Y" not in os.environ
     meshes, states, dts = _members(ccs, sizes=(48, 24, 32))
     with ccs.Ensemble(meshes, dts, precision="fp32") as ens:
         for k, s in enumerate(states):
@@ -63,9 +66,24 @@ def test_autotuned_members_equal_single_runs_at_the_chosen_tile(ccs, monkeypatch
         assert choice["tile_y"] in (8, 16, 32) and choice["persistent"] in (0, 1) and choice["stage_ms"] > 0
         ens.step(17)
         got = [ens.get_state(k) for k in range(len(ens))]
-    monkeypatch.setenv("CCS_SPLIT_TILE_Y", str(choice["tile_y"]))
     for k in range(len(meshes)):
         ref, _ = _single(ccs, meshes[k], states[k], dts[k], "fp32", "RK3_TVD", 17)
+        for f in ("phiG", "u", "v"):
+            assert np.array_equal(getattr(got[k], f), getattr(ref, f)), (k, f)
+
+
+@pytest.mark.parametrize("tile_y,persistent", [(8, 0), (32, 0), (8, 1), (32, 1)])
+def test_members_at_every_tile_height_equal_single_runs(ccs, tile_y, persistent):
+    assert "CCS_SPLIT_TILE_Y" not in os.environ
+    meshes, states, dts = _members(ccs, sizes=(40, 24, 100))
+    print(f"fp32 ensemble, {tile_y}-row tiles, persistent={persistent}: 18 steps", flush=True)
+    with ccs.Ensemble(meshes, dts, precision="fp32", tile_y=tile_y, persistent=persistent) as ens:
+        for k, s in enumerate(states):
+            ens.set_state(k, s)
+        ens.step(18)
+        got = [ens.get_state(k) for k in range(len(ens))]
+    for k in range(len(meshes)):
+        ref, _ = _single(ccs, meshes[k], states[k], dts[k], "fp32", "RK3_TVD", 18)
         for f in ("phiG", "u", "v"):
             assert np.array_equal(getattr(got[k], f), getattr(ref, f)), (k, f)
 

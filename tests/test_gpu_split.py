@@ -173,6 +173,28 @@ def test_split_step_grouping_does_not_change_results(ccs, scheme):
         assert d == out[0][1]
 
 
+@pytest.mark.parametrize("precision,scheme,n", [("fp32", "RK3_TVD", 40), ("fp32", "RK3_TVD", 100),
+                                                ("fp32", "RK4", 40), ("fp64_fast", "RK3_TVD", 40)])
+def test_split_results_do_not_depend_on_the_tile_height(ccs, monkeypatch, precision, scheme, n):
+    # a face value is computed by the tile on either side of it, by the
+    # interior windows of one and the boundary windows of the other; both
+    # round alike, so 8-, 16- and 32-row tiles give the same state bit for
+    # bit (n = 100: partial tiles in both directions)
+    mesh, s0, dt, _ = _setup(ccs, n=n, seed=8)
+    out = []
+    for ty in (16, 8, 32):
+        monkeypatch.setenv("CCS_SPLIT_TILE_Y", str(ty))
+        print(f"{precision} {scheme} C{n}: 21 steps with {ty}-row tiles", flush=True)
+        with ccs.DynamicalCore(mesh, dt, precision=precision, scheme=scheme) as core:
+            core.set_state(s0)
+            core.step(21)
+            out.append(core.get_state())
+    monkeypatch.delenv("CCS_SPLIT_TILE_Y")
+    for s in out[1:]:
+        for k in ("phiG", "u", "v"):
+            assert np.array_equal(getattr(s, k), getattr(out[0], k)), k
+
+
 def test_split_halo_reference_follows_its_source_cell(ccs):
     # An uploaded state whose halo phiG differs from the interior cell it
     # copies (here by 1e-3 relative).  Both paths read the halo as uploaded in
