@@ -132,6 +132,31 @@ def test_one_day_c32_normalised_l2(ccs, orc):
     assert l2 <= 1e-12
 
 
+def test_reference_namelist_configuration_one_day(ccs, orc):
+    """The configuration of the reference's shipped namelist
+    (dynamic_core/run/namelist.input): RK4, Williamson case 2, halo 4 and a
+    fixed dt = 100 s, here at C48 for one simulated day (864 steps), equal
+    to the oracle bit for bit away from the never-filled halo corners."""
+    mesh = ccs.Mesh(48, halo=4)
+    s0 = ccs.init_case(mesh, case_num=2)
+    dt, nsteps = 100.0, 864
+    print(f"reference namelist: C48 halo 4, case 2, RK4, {nsteps} steps of dt={dt}", flush=True)
+    with ccs.DynamicalCore(mesh, dt, scheme="RK4") as core:
+        core.set_state(s0)
+        core.step(nsteps)
+        s = core.get_state()
+        m, e = core.diagnostics()
+    o = orc.Oracle(mesh, s0, dt)
+    o.steps(nsteps, scheme=1)
+    mask = ~mesh.corner_mask().ravel()
+    for k in ("phiG", "u", "v", "phi"):
+        same = np.array_equal(getattr(s, k)[mask], o.a[k][mask])
+        print(f"  {k}: bitwise equal {same}, max rel diff {_rel(getattr(s, k), o.a[k], mask):.3e}", flush=True)
+        assert same, k
+    _, _, om, oe = o.diagnostics()
+    assert abs(m - om) <= 1e-14 * abs(om) and abs(e - oe) <= 1e-14 * abs(oe)
+
+
 @pytest.mark.slow
 def test_one_day_c180_matches_cpu_oracle(ccs, orc):
     """The drop-in FP64 path at C180 (the benchmark recipe: a = 6.371e6 m,
