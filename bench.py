@@ -121,6 +121,22 @@ def time_path(ccs, mesh, s0, dt, precision, steps, warmup, spatial_op="weno", se
     return core, wall, ev_total, kern_ms
 
 
+def field_norms(ccs, mesh, st, ref):
+    """Normalised, area-weighted L1/L2/Linf of h, zonal and meridional wind
+    of state st against ref (interior cells)."""
+    area = mesh.interior(mesh.areaCell)
+    out = {}
+    for name, f in (("h", lambda s: s.phi / GRAVITY), ("u", lambda s: s.zonal), ("v", lambda s: s.merid)):
+        l1, l2, linf = ccs.error_norms(mesh.interior(f(st)), mesh.interior(f(ref)), area)
+        out[name] = {"L1": l1, "L2": l2, "Linf": linf}
+    return out
+
+
+def drift(d0, d1):
+    """Relative mass and energy drift from diagnostics (mass, energy) d0 to d1."""
+    return {"mass": (d1[0] - d0[0]) / d0[0], "energy": (d1[1] - d0[1]) / d0[1]}
+
+
 def host_cores():
     cores = os.cpu_count() or 1
     try:
@@ -349,15 +365,24 @@ def main():
         # beside the method's own floor (an FP64 run whose initial state is
         # perturbed at FP32 rounding level, 2^-24 relative, halo-consistent).
         s32 = core.get_state()
+        d32 = core.diagnostics()
         core.close()
         nadv = core.advanced
+        # the FP32 handle's own diagnostics of the uploaded state (its winds
+        # are the FP32 casts) are the start of its drift
+        with ccs.DynamicalCore(mesh, dt, precision="fp32", spatial_op=args.spatial_op) as c:
+            c.set_state(s0)
+            d32_0 = c.diagnostics()
         sp = ccs.perturb_state(mesh, s0, 2.0 ** -24)
-        states = {}
+        states, drift64 = {}, None
         for name, st0 in (("ref", s0), ("pert", sp)):
             c = ccs.DynamicalCore(mesh, dt, precision="fp64", spatial_op=args.spatial_op)
             c.set_state(st0)
+            d0 = c.diagnostics()
             c.step(nadv)
             states[name] = c.get_state()
+            if name == "ref":
+                drift64 = drift(d0, c.diagnostics())
             c.close()
         area = mesh.interior(mesh.areaCell)
         href = mesh.interior(states["ref"].phi) / GRAVITY
@@ -366,6 +391,11 @@ def main():
                                                              area)[1]
         out["l2_h_compared_after_steps"] = nadv
         out["l2_h_compared_after_hours"] = nadv * dt / 3600.0
+        # normalised, area-weighted L1/L2/Linf of h, zonal and meridional
+        # wind against the FP64 parity run, and relative mass/energy drift
+        out["errors_vs_fp64"] = {"fp32": field_norms(ccs, mesh, s32, states["ref"]),
+                                 "fp64_perturbed_2^-24": field_norms(ccs, mesh, states["pert"], states["ref"])}
+        out["drift"] = {"fp32": drift(d32_0, d32), "fp64": drift64}
     else:
         core.close()
     if not args.no_ensemble:

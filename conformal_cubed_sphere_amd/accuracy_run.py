@@ -69,6 +69,22 @@ def _run(mesh, s0, dt, nsteps, precision, with_companions, chunk, label, marks=(
                         "cell_updates_per_s": 6 * mesh.n * mesh.n * nsteps / wall, "wall_s": wall}
 
 
+def _drift_series(mesh, s0, dt, precision, marks):
+    """Relative mass and energy drift after each step count in marks."""
+    out = {"mass": [], "energy": []}
+    with DynamicalCore(mesh, dt, precision=precision) as core:
+        core.set_state(s0)
+        m0, e0 = core.diagnostics()
+        done = 0
+        for m in marks:
+            core.step(m - done)
+            done = m
+            m1, e1 = core.diagnostics()
+            out["mass"].append((m1 - m0) / m0)
+            out["energy"].append((e1 - e0) / e0)
+    return out
+
+
 def _norms(mesh, st, ref):
     area = mesh.interior(mesh.areaCell)
     out = {}
@@ -94,6 +110,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=4000)
     ap.add_argument("--only", choices=["fp32"], default=None,
                     help="run only the FP32 path (drift, stability and throughput; no error norms)")
+    ap.add_argument("--drift-steps", default="",
+                    help="comma-separated step counts: relative mass and energy drift of FP32, FP64 parity, "
+                         "FP64 from the 2^-24-perturbed state and the split kernel in FP64 at each (no error norms)")
     args = ap.parse_args()
 
     mesh = Mesh(args.n, halo=3, radius=6.371e6, omega=7.292e-5)
@@ -102,6 +121,15 @@ def main():
     else:
         s0 = init_case(mesh, case_num=2, gravity=GRAVITY)
     dt, nsteps = cfl_time_step(mesh, s0, 0.5, args.days * 86400.0)
+    if args.drift_steps:
+        marks = sorted({int(x) for x in args.drift_steps.split(",")})
+        out = {"n": args.n, "case": args.case, "dt_s": dt, "steps": marks, "drift": {}}
+        for name, prec, st in (("fp32", "fp32", s0), ("fp64", "fp64", s0),
+                               ("fp64_perturbed_2^-24", "fp64", perturb_state(mesh, s0, 2.0 ** -24)),
+                               ("fp64_fast", "fp64_fast", s0)):
+            out["drift"][name] = _drift_series(mesh, st, dt, prec, marks)
+        print(json.dumps(out), flush=True)
+        return
     if args.only == "fp32":
         f32, _, f32_info = _run(mesh, s0, dt, nsteps, "fp32", True, args.chunk, "fp32")
         h = mesh.interior(f32.phi) / GRAVITY

@@ -627,17 +627,18 @@ __device__ __forceinline__ void stage_tile(const SplitArgs<T>& a, const SplitSta
     pn = pre.b0 + a.dts * tp;
     un = pre.b1 + a.dts * tu;
     vn = pre.b2 + a.dts * tv;
-  } else if (KIND == kRK3s2) {
-    const T w = T(0.25) * a.dts;
-    pn = T(0.75) * pre.b0 + T(0.25) * pre.c0 + w * tp;
-    un = T(0.75) * pre.b1 + T(0.25) * pre.c1 + w * tu;
-    vn = T(0.75) * pre.b2 + T(0.25) * pre.c2 + w * tv;
-  } else if (KIND == kRK3s3) {
-    const T c3 = T(1) / T(3), c23 = T(2) / T(3);
-    const T w = c23 * a.dts;
-    pn = c3 * pre.b0 + c23 * pre.c0 + w * tp;
-    un = c3 * pre.b1 + c23 * pre.c1 + w * tu;
-    vn = c3 * pre.b2 + c23 * pre.c2 + w * tv;
+  } else if (KIND == kRK3s2 || KIND == kRK3s3) {
+    // q2 = 3/4 q + 1/4 (q1 + dt L) and q' = 1/3 q + 2/3 (q2 + dt L)
+    // (time_scheme_mod.F90:85-115) as q + c ((q_k - q) + dt L): the base
+    // state q enters with the exact coefficient 1 and c rounds only the
+    // increment.  (In FP32, fl(1/3) + fl(2/3) = 1 + 2^-25, so the
+    // weighted-sum form scaled the winds U, V by that factor every step:
+    // total energy grew by 1.0e-6 in 8192 steps at C720 while the FP64 path
+    // lost 2.6e-8.)
+    const T c = (KIND == kRK3s2) ? T(0.25) : T(2) / T(3);
+    pn = pre.b0 + c * ((pre.c0 - pre.b0) + a.dts * tp);
+    un = pre.b1 + c * ((pre.c1 - pre.b1) + a.dts * tu);
+    vn = pre.b2 + c * ((pre.c2 - pre.b2) + a.dts * tv);
   } else if (KIND == kRK4a) {
     sto(a.tend[0], io, tp);
     sto(a.tend[1], io, tu);
