@@ -85,6 +85,36 @@ def _drift_series(mesh, s0, dt, precision, marks):
     return out
 
 
+def _survival(mesh, s0, dt, nsteps, precision, chunk, step0=0, record=False):
+    """Step from s0 in chunks, checking that h and the zonal wind stay finite;
+    the largest |u| and smallest h seen, and when the state went non-finite."""
+    out = {"blew_up_after_step": None, "blew_up_by_hours": None, "umax": 0.0, "hmin": float("inf")}
+    rows = []
+    with DynamicalCore(mesh, dt, precision=precision) as core:
+        core.set_state(s0)
+        done = 0
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            core.step(k)
+            done += k
+            st = core.get_state()
+            h = mesh.interior(st.phi) / GRAVITY
+            u = np.abs(mesh.interior(st.zonal))
+            finite = bool(np.all(np.isfinite(h)) and np.all(np.isfinite(u)))
+            if record:
+                rows.append({"step": step0 + done, "finite": finite, "umax": float(np.nanmax(u)),
+                             "hmin": float(np.nanmin(h))})
+            if not finite:
+                out["blew_up_after_step"] = step0 + done
+                out["blew_up_by_hours"] = (step0 + done) * dt / 3600.0
+                break
+            out["umax"] = max(out["umax"], float(np.max(u)))
+            out["hmin"] = min(out["hmin"], float(np.min(h)))
+    if record:
+        out["rows"] = rows
+    return out
+
+
 def _norms(mesh, st, ref):
     area = mesh.interior(mesh.areaCell)
     out = {}
@@ -113,6 +143,16 @@ def main():
     ap.add_argument("--drift-steps", default="",
                     help="comma-separated step counts: relative mass and energy drift of FP32, FP64 parity, "
                          "FP64 from the 2^-24-perturbed state and the split kernel in FP64 at each (no error norms)")
+    ap.add_argument("--cfl", type=float, default=0.5, help="Courant number of the time step (initial state)")
+    ap.add_argument("--survival", default="",
+                    help="comma-separated k (0: unperturbed): does the run from the state perturbed by 2^-k stay "
+                         "finite for --days (checked every --chunk steps)?  Precision from --precision")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64_fast", "fp64"],
+                    help="precision of the --survival runs")
+    ap.add_argument("--branch-at", type=int, default=0,
+                    help="run FP32 this many steps, then continue that state in FP32, FP64-fast and FP64 parity "
+                         "for --branch-steps steps, checking every 50 steps")
+    ap.add_argument("--branch-steps", type=int, default=400)
     args = ap.parse_args()
 
     mesh = Mesh(args.n, halo=3, radius=6.371e6, omega=7.292e-5)
@@ -120,7 +160,29 @@ def main():
         s0 = init_case(mesh, case_num=7, seed=args.seed, n_bumps=8, mountain=True, gravity=GRAVITY)
     else:
         s0 = init_case(mesh, case_num=2, gravity=GRAVITY)
-    dt, nsteps = cfl_time_step(mesh, s0, 0.5, args.days * 86400.0)
+    dt, nsteps = cfl_time_step(mesh, s0, args.cfl, args.days * 86400.0)
+    if args.survival:
+        out = {"n": args.n, "case": args.case, "days": args.days, "cfl": args.cfl, "dt_s": dt, "steps": nsteps,
+               "precision": args.precision, "runs": []}
+        for k in [int(x) for x in args.survival.split(",")]:
+            st = s0 if k == 0 else perturb_state(mesh, s0, 2.0 ** -k)
+            out["runs"].append({"perturbation": f"2^-{k}" if k else "none",
+                                **_survival(mesh, st, dt, nsteps, args.precision, args.chunk)})
+            print(json.dumps(out["runs"][-1]), file=sys.stderr, flush=True)
+        print(json.dumps(out), flush=True)
+        return
+    if args.branch_at:
+        out = {"n": args.n, "case": args.case, "cfl": args.cfl, "dt_s": dt, "branch_at": args.branch_at,
+               "continued": {}}
+        with DynamicalCore(mesh, dt, precision="fp32") as core:
+            core.set_state(s0)
+            core.step(args.branch_at)
+            sb = core.get_state()
+        for prec in ("fp32", "fp64_fast", "fp64"):
+            out["continued"][prec] = _survival(mesh, sb, dt, args.branch_steps, prec, 50, args.branch_at,
+                                               record=True)
+        print(json.dumps(out), flush=True)
+        return
     if args.drift_steps:
         marks = sorted({int(x) for x in args.drift_steps.split(",")})
         out = {"n": args.n, "case": args.case, "dt_s": dt, "steps": marks, "drift": {}}
