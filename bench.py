@@ -214,8 +214,12 @@ def ensemble_bench(ccs, members, steps, warmup, settle_s=0.3):
     # the GPU's clocks ramp up over the first tens of milliseconds of
     # sustained work and drop while it idles (as during uploads and graph
     # captures): untimed batched steps of a second copy of the ensemble for
-    # settle_s, then the timed steps at once
-    warm = ccs.Ensemble([r[0] for r in runs], [r[2] for r in runs], precision="fp32")
+    # settle_s, then the timed steps at once.  The second copy runs the other
+    # launch grid at the chosen tile height (the persistent grid, whose
+    # workgroups claim tiles from per-XCD counters, or one workgroup per
+    # tile), and is timed after the batched steps on the same warm clocks.
+    other = {"tile_y": choice["tile_y"], "persistent": 1 - choice["persistent"]}
+    warm = ccs.Ensemble([r[0] for r in runs], [r[2] for r in runs], precision="fp32", **other)
     for k, r in enumerate(runs):
         warm.set_state(k, r[1])
     t0 = time.perf_counter()
@@ -226,6 +230,10 @@ def ensemble_bench(ccs, members, steps, warmup, settle_s=0.3):
     ens.step(steps)
     ens.synchronize()
     wall_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    warm.step(steps)
+    warm.synchronize()
+    other["value"] = cells * steps / (time.perf_counter() - t0)
     warm.close()
 
     cores = []
@@ -271,7 +279,7 @@ def ensemble_bench(ccs, members, steps, warmup, settle_s=0.3):
     return {"members": members, "members_per_n": counts, "interior_cells": cells, "steps": steps,
             "value_batched": cells * steps / wall_b, "value_serial": cells * steps / wall_s,
             "unit": "member cell-updates/s", "speedup_batched_over_serial": wall_s / wall_b,
-            "autotune": choice, "precision": "fp32",
+            "autotune": choice, "other_launch_grid": other, "precision": "fp32",
             "l2_h_vs_fp64_median": float(np.median(l2)), "l2_h_vs_fp64_max": float(np.max(l2)),
             "l2_h_fp64_perturbed_median": float(np.median(floor)), "l2_h_fp64_perturbed_max": float(np.max(floor)),
             "mass_drift_max": float(np.max(drift)),
