@@ -34,7 +34,7 @@ import time
 
 import numpy as np
 
-from .core import GRAVITY, DynamicalCore, Mesh, cfl_time_step, error_norms, init_case, perturb_state
+from .core import GRAVITY, CcsError, DynamicalCore, Mesh, cfl_time_step, error_norms, init_case, perturb_state
 
 
 def _run(mesh, s0, dt, nsteps, precision, with_companions, chunk, label, marks=(), ref_h=None):
@@ -62,7 +62,10 @@ def _run(mesh, s0, dt, nsteps, precision, with_companions, chunk, label, marks=(
                     series[stop] = h.copy()
                 else:
                     series[stop] = error_norms(h, ref_h[stop], area)[1]
-        m1, e1 = core.diagnostics()
+        try:
+            m1, e1 = core.diagnostics()
+        except CcsError:  # the state went non-finite: the drift and the norms come out NaN
+            m1 = e1 = float("nan")
         st = core.get_state()
     wall = time.perf_counter() - t0
     return st, series, {"mass_drift": (m1 - m0) / m0, "energy_drift": (e1 - e0) / e0,
@@ -79,7 +82,10 @@ def _drift_series(mesh, s0, dt, precision, marks):
         for m in marks:
             core.step(m - done)
             done = m
-            m1, e1 = core.diagnostics()
+            try:
+                m1, e1 = core.diagnostics()
+            except CcsError:  # non-finite state
+                m1 = e1 = float("nan")
             out["mass"].append((m1 - m0) / m0)
             out["energy"].append((e1 - e0) / e0)
     return out
@@ -207,7 +213,8 @@ def main():
     out = {"n": args.n, "case": args.case, "days": args.days, "steps": nsteps, "dt_s": dt,
            "reference": args.reference, "snapshot_hours": [m * dt / 3600.0 for m in marks],
            "fp32_vs_fp64": _norms(mesh, f32, ref), "fp32_l2_h_series": [f32_l2[m] for m in marks],
-           "fp32": f32_info, args.reference: ref_info, "perturbed": {}}
+           "fp32": f32_info, args.reference: ref_info, "perturbed": {},
+           "fp32_finite": bool(np.all(np.isfinite(mesh.interior(f32.phi))))}
     for k in [int(x) for x in args.perturb.split(",") if x.strip()]:
         sp = perturb_state(mesh, s0, 2.0 ** -k)
         pert, pert_l2, pert_info = _run(mesh, sp, dt, nsteps, args.reference, True, args.chunk,
